@@ -1,0 +1,10 @@
+"""MI355X-native H-matrix arithmetic (HMat2d / HMat3d hot path of dmhm).
+
+The compute lives in libdmhm_amd.so (hand-written gfx950 HIP kernels behind
+the C-ABI in include/dmhm_amd.h); this package is the Python host mirror of
+the reference interface plus the synthetic workload generators.
+"""
+from ._lib import (DMHM_F64, DMHM_F32, DmhmError, NonConformalError, NotConvergedError,  # noqa: F401
+                   init, load)
+from .hmat import HMat2d, HMat3d, EstimateTwoNorm, build_natural_to_hierarchical_map  # noqa: F401
+from . import workloads  # noqa: F401

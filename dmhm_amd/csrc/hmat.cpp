@@ -1,0 +1,962 @@
+// Host runtime and C-ABI of libdmhm_amd.so: device storage of H-matrices,
+// plan caches, and the operations of include/dmhm_amd.h.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dmhm_amd.h"
+#include "geometry.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace dmhm {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                       \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            throw Error(e_ == hipErrorOutOfMemory ? DMHM_ENOMEM : DMHM_EHIP,              \
+                        std::string(#x) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+static thread_local std::string g_lastError;
+
+// ---------------------------------------------------------------------------
+// Library state: one device, one stream (default priority), a growing
+// workspace and per-geometry plan caches.
+// ---------------------------------------------------------------------------
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b) {
+        if (b <= bytes) return;
+        if (ptr) HIPCHECK(hipFree(ptr));
+        ptr = nullptr; bytes = 0;
+        HIPCHECK(hipMalloc(&ptr, b));
+        bytes = b;
+    }
+    void release() { if (ptr) (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
+};
+
+struct GeoCtx;
+
+struct Library {
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DeviceBuffer ws;            // plan workspace
+    DeviceBuffer stageX, stageY;
+    double* dScalars = nullptr; // power-iteration scalars
+    std::vector<std::weak_ptr<GeoCtx>> geos;
+    std::mutex mu;
+    int nextLayoutId = 100;
+};
+static Library g_lib;
+
+static void require_ready()
+{
+    if (!g_lib.ready) throw Error(DMHM_ENODEV, "dmhm_init has not been called");
+}
+
+struct DeviceTables {        // small index tables for per-leaf kernels
+    void* ptr = nullptr;
+    ~DeviceTables() { if (ptr) (void)hipFree(ptr); }
+};
+
+struct GeoCtx {
+    Geometry geo;
+    std::map<std::string, std::unique_ptr<Plan>> plans;
+    std::map<int, std::shared_ptr<Layout>> stdLayouts;      // by maxRank
+    // diagonal dense leaves (t == s) for AddConstantToDiagonal, per layout id
+    std::map<int, std::pair<int64_t*, int*>> diagTables;
+    std::map<int, int> diagCount;
+    GeoCtx(int dim, const int* sizes, int L, bool strong) : geo(dim, sizes, L, strong) {}
+    ~GeoCtx() {
+        for (auto& kv : diagTables) { (void)hipFree(kv.second.first); (void)hipFree(kv.second.second); }
+    }
+};
+
+static std::shared_ptr<GeoCtx> get_geo(int dim, const int* sizes, int L, bool strong)
+{
+    std::lock_guard<std::mutex> lk(g_lib.mu);
+    for (auto& w : g_lib.geos) {
+        auto g = w.lock();
+        if (!g) continue;
+        const Geometry& G = g->geo;
+        bool same = G.dim == dim && G.numLevels == L && G.strong == strong;
+        for (int a = 0; a < dim && same; ++a) same = G.sizes[a] == sizes[a];
+        if (same) return g;
+    }
+    auto g = std::make_shared<GeoCtx>(dim, sizes, L, strong);
+    g_lib.geos.push_back(g);
+    return g;
+}
+
+static std::shared_ptr<Layout> std_layout(GeoCtx& gc, int maxRank)
+{
+    auto it = gc.stdLayouts.find(maxRank);
+    if (it != gc.stdLayouts.end()) return it->second;
+    std::vector<int> caps(gc.geo.lrBlocks.size(), std::max(maxRank, 1));
+    auto L = std::make_shared<Layout>(Layout::make(gc.geo, caps, maxRank + 1));
+    gc.stdLayouts[maxRank] = L;
+    return L;
+}
+
+} // namespace dmhm
+
+using namespace dmhm;
+
+struct dmhm_hmat_s {
+    std::shared_ptr<GeoCtx> gc;
+    std::shared_ptr<Layout> lay;
+    int prec = DMHM_F64;
+    int maxRank = 0;
+    void* vals = nullptr;
+    int* ranks = nullptr;
+    size_t esize() const { return prec == DMHM_F64 ? 8 : 4; }
+    ~dmhm_hmat_s() {
+        if (vals) (void)hipFree(vals);
+        if (ranks) (void)hipFree(ranks);
+    }
+    const Geometry& geo() const { return gc->geo; }
+};
+
+namespace dmhm {
+
+static dmhm_hmat_s* alloc_hmat(std::shared_ptr<GeoCtx> gc, std::shared_ptr<Layout> lay, int prec,
+                               int maxRank)
+{
+    std::unique_ptr<dmhm_hmat_s> h(new dmhm_hmat_s());
+    h->gc = gc; h->lay = lay; h->prec = prec; h->maxRank = maxRank;
+    const size_t bytes = std::max<size_t>(lay->totalElems() * h->esize(), 256);
+    HIPCHECK(hipMalloc(&h->vals, bytes));
+    HIPCHECK(hipMemsetAsync(h->vals, 0, bytes, g_lib.stream));
+    const size_t rb = std::max<size_t>(lay->nLowRank() * sizeof(int), 4);
+    HIPCHECK(hipMalloc((void**)&h->ranks, rb));
+    HIPCHECK(hipMemsetAsync(h->ranks, 0, rb, g_lib.stream));
+    return h.release();
+}
+
+static Bases bases_for(dmhm_hmat_s* A, dmhm_hmat_s* B, dmhm_hmat_s* C, void* X, void* Y)
+{
+    Bases b{};
+    if (A) { b.val[SLOT_A] = A->vals; b.rank[SLOT_A] = A->ranks; }
+    if (B) { b.val[SLOT_B] = B->vals; b.rank[SLOT_B] = B->ranks; }
+    if (C) { b.val[SLOT_C] = C->vals; b.rank[SLOT_C] = C->ranks; }
+    b.val[SLOT_WS] = g_lib.ws.ptr;
+    b.val[SLOT_X] = X; b.val[SLOT_Y] = Y;
+    return b;
+}
+
+template <typename T>
+static void run_plan(Plan& p, const Bases& b, T alpha, T beta, TruncPolicy pol, uint64_t seed)
+{
+    hipStream_t s = g_lib.stream;
+    for (auto& ph : p.phases) {
+        switch (ph.kind) {
+        case PH_TILE: launch_tile_phase<T>(ph, b, alpha, beta, s); break;
+        case PH_QR: launch_qr_phase<T>(ph, b, s); break;
+        case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
+        case PH_RNG: launch_rng_phase<T>(ph, b, seed, s); break;
+        default: break;
+        }
+    }
+}
+
+static Plan& get_plan(GeoCtx& gc, const std::string& key, const std::function<Plan*()>& make)
+{
+    auto it = gc.plans.find(key);
+    if (it != gc.plans.end()) return *it->second;
+    std::unique_ptr<Plan> p(make());
+    p->upload();
+    Plan& ref = *p;
+    gc.plans[key] = std::move(p);
+    return ref;
+}
+
+static void ensure_ws(const Plan& p, size_t esize)
+{
+    g_lib.ws.ensure(std::max<size_t>(p.wsElems * esize, 256));
+}
+
+// ---------------------------------------------------------------------------
+// Operations
+// ---------------------------------------------------------------------------
+static void multiply_dense(dmhm_hmat_s* h, bool trans, double alpha, const void* X, int ldx, int k,
+                           double beta, void* Y, int ldy)
+{
+    GeoCtx& gc = *h->gc;
+    const std::string key = "mv:" + std::to_string(h->lay->id) + ":" + std::to_string(trans) + ":" +
+                            std::to_string(k) + ":" + std::to_string(ldx) + ":" + std::to_string(ldy);
+    Plan& p = get_plan(gc, key, [&] { Planner pl(gc.geo); return pl.matvec(*h->lay, trans, k, ldx, ldy); });
+    ensure_ws(p, h->esize());
+    Bases b = bases_for(h, nullptr, nullptr, const_cast<void*>(X), Y);
+    if (h->prec == DMHM_F64) run_plan<double>(p, b, alpha, beta, {0, 0}, 0);
+    else run_plan<float>(p, b, (float)alpha, (float)beta, {0, 0}, 0);
+}
+
+static dmhm_hmat_s* multiply_hmat(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, const dmhm_hh_opts& o)
+{
+    if (A->gc != B->gc) throw Error(DMHM_ENONCONF, "Attempted nonconformal matrix-matrix multiply");
+    if (A->prec != B->prec) throw Error(DMHM_EINVAL, "operands must share a precision");
+    GeoCtx& gc = *A->gc;
+    const int maxRank = A->maxRank;
+    auto LC = std_layout(gc, maxRank);
+    std::unique_ptr<dmhm_hmat_s> C(alloc_hmat(A->gc, LC, A->prec, maxRank));
+    const int over = o.oversample > 0 ? o.oversample : 4;
+    const std::string key = "hh:" + std::to_string(A->lay->id) + ":" + std::to_string(B->lay->id) + ":" +
+                            std::to_string(LC->id) + ":" + std::to_string(maxRank) + ":" +
+                            std::to_string(over);
+    Plan& p = get_plan(gc, key, [&] {
+        Planner pl(gc.geo);
+        return pl.hhProduct(*A->lay, *B->lay, *LC, maxRank, over);
+    });
+    ensure_ws(p, A->esize());
+    Bases b = bases_for(A, B, C.get(), nullptr, nullptr);
+    TruncPolicy pol{o.tolMode, o.relTol};
+    if (A->prec == DMHM_F64) run_plan<double>(p, b, alpha, 0.0, pol, o.seed);
+    else run_plan<float>(p, b, (float)alpha, 0.0f, pol, o.seed);
+    return C.release();
+}
+
+static dmhm_hmat_s* copy_hmat(dmhm_hmat_s* s, int prec)
+{
+    std::unique_ptr<dmhm_hmat_s> d(alloc_hmat(s->gc, s->lay, prec, s->maxRank));
+    const int64_t n = s->lay->totalElems();
+    if (prec == s->prec) HIPCHECK(hipMemcpyAsync(d->vals, s->vals, n * s->esize(), hipMemcpyDeviceToDevice, g_lib.stream));
+    else if (s->prec == DMHM_F64) launch_convert<double, float>((const double*)s->vals, (float*)d->vals, n, g_lib.stream);
+    else launch_convert<float, double>((const float*)s->vals, (double*)d->vals, n, g_lib.stream);
+    HIPCHECK(hipMemcpyAsync(d->ranks, s->ranks, s->lay->nLowRank() * sizeof(int), hipMemcpyDeviceToDevice,
+                            g_lib.stream));
+    return d.release();
+}
+
+template <typename T>
+static void upload(std::vector<T>& v, T** d)
+{
+    HIPCHECK(hipMalloc((void**)d, std::max<size_t>(v.size(), 1) * sizeof(T)));
+    if (!v.empty()) HIPCHECK(hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+static dmhm_hmat_s* transpose_hmat(dmhm_hmat_s* s)
+{
+    const Geometry& g = s->geo();
+    const Layout& Ls = *s->lay;
+    // Destination capacities: block b takes its partner's capacity.
+    std::vector<int> caps(g.lrBlocks.size());
+    bool same = true;
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        const int partner = g.blocks[g.blocks[g.lrBlocks[i]].transposeBlock].lrIndex;
+        caps[i] = Ls.cap[partner];
+        same = same && caps[i] == Ls.cap[i];
+    }
+    std::shared_ptr<Layout> Ld = s->lay;
+    if (!same) {
+        std::lock_guard<std::mutex> lk(g_lib.mu);
+        Ld = std::make_shared<Layout>(Layout::make(g, caps, g_lib.nextLayoutId++));
+    }
+    std::unique_ptr<dmhm_hmat_s> d(alloc_hmat(s->gc, Ld, s->prec, s->maxRank));
+    std::vector<int64_t> so, dof; std::vector<int> mm, nn;
+    for (size_t i = 0; i < g.denseBlocks.size(); ++i) {
+        const int b = g.denseBlocks[i];
+        const int pb = g.blocks[b].transposeBlock;
+        so.push_back(Ls.denseOff[g.blocks[pb].denseIndex]);
+        dof.push_back(Ld->denseOff[i]);
+        mm.push_back(g.tbox(pb).size); nn.push_back(g.sbox(pb).size);
+    }
+    std::vector<int64_t> su, sv, du, dv; std::vector<int> mr, nr, cp, perm;
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        const int b = g.lrBlocks[i];
+        const int pl = g.blocks[g.blocks[b].transposeBlock].lrIndex;
+        su.push_back(Ls.uOff[pl]); sv.push_back(Ls.vOff[pl]);
+        du.push_back(Ld->uOff[i]); dv.push_back(Ld->vOff[i]);
+        mr.push_back(g.tbox(b).size); nr.push_back(g.sbox(b).size); cp.push_back(caps[i]);
+        perm.push_back(pl);
+    }
+    int64_t *dso, *ddo, *dsu, *dsv, *ddu, *ddv; int *dmm, *dnn, *dmr, *dnr, *dcp, *dperm;
+    upload(so, &dso); upload(dof, &ddo); upload(mm, &dmm); upload(nn, &dnn);
+    upload(su, &dsu); upload(sv, &dsv); upload(du, &ddu); upload(dv, &ddv);
+    upload(mr, &dmr); upload(nr, &dnr); upload(cp, &dcp); upload(perm, &dperm);
+    hipStream_t st = g_lib.stream;
+    if (s->prec == DMHM_F64) {
+        launch_transpose_dense<double>((const double*)s->vals, (double*)d->vals, dso, ddo, dmm, dnn, (int)so.size(), st);
+        launch_swap_lowrank<double>((const double*)s->vals, (double*)d->vals, dsu, dsv, ddu, ddv, dmr, dnr, dcp,
+                                    (int)su.size(), st);
+    } else {
+        launch_transpose_dense<float>((const float*)s->vals, (float*)d->vals, dso, ddo, dmm, dnn, (int)so.size(), st);
+        launch_swap_lowrank<float>((const float*)s->vals, (float*)d->vals, dsu, dsv, ddu, ddv, dmr, dnr, dcp,
+                                   (int)su.size(), st);
+    }
+    launch_permute_int(s->ranks, d->ranks, dperm, (int)perm.size(), st);
+    HIPCHECK(hipStreamSynchronize(st));
+    for (void* p : {(void*)dso, (void*)ddo, (void*)dsu, (void*)dsv, (void*)ddu, (void*)ddv, (void*)dmm,
+                    (void*)dnn, (void*)dmr, (void*)dnr, (void*)dcp, (void*)dperm})
+        (void)hipFree(p);
+    return d.release();
+}
+
+static void scale_hmat(dmhm_hmat_s* h, double alpha)
+{
+    const Layout& L = *h->lay;
+    hipStream_t st = g_lib.stream;
+    if (alpha == 0.0) {
+        // hmat_tools::Scale(0, LowRank) drops the rank (hmat_tools.hpp:1644-1648).
+        HIPCHECK(hipMemsetAsync(h->vals, 0, L.totalElems() * h->esize(), st));
+        launch_fill_int(h->ranks, L.nLowRank(), 0, st);
+        return;
+    }
+    const int64_t n = L.denseElems + L.uElems;     // dense pool and U pool are contiguous
+    if (h->prec == DMHM_F64) launch_scale<double>((double*)h->vals, n, alpha, st);
+    else launch_scale<float>((float*)h->vals, n, (float)alpha, st);
+}
+
+static void add_diag(dmhm_hmat_s* h, double alpha)
+{
+    GeoCtx& gc = *h->gc;
+    const Geometry& g = gc.geo;
+    const int id = h->lay->id;
+    if (!gc.diagTables.count(id)) {
+        std::vector<int64_t> offs; std::vector<int> sizes;
+        for (size_t i = 0; i < g.denseBlocks.size(); ++i) {
+            const Block& b = g.blocks[g.denseBlocks[i]];
+            if (b.t == b.s) { offs.push_back(h->lay->denseOff[i]); sizes.push_back(g.tbox(g.denseBlocks[i]).size); }
+        }
+        int64_t* d1; int* d2;
+        upload(offs, &d1); upload(sizes, &d2);
+        gc.diagTables[id] = {d1, d2};
+        gc.diagCount[id] = (int)offs.size();
+    }
+    auto t = gc.diagTables[id];
+    const int cnt = gc.diagCount[id];
+    if (h->prec == DMHM_F64) launch_add_diag<double>((double*)h->vals, t.first, t.second, cnt, alpha, g_lib.stream);
+    else launch_add_diag<float>((float*)h->vals, t.first, t.second, cnt, (float)alpha, g_lib.stream);
+}
+
+static void set_to_random(dmhm_hmat_s* h, uint64_t seed)
+{
+    const Layout& L = *h->lay;
+    hipStream_t st = g_lib.stream;
+    // Dense leaves, then U and V at rank = maxRank (SetToRandom-incl.hpp:34-48);
+    // columns beyond maxRank (import capacities) stay zero.
+    if (h->prec == DMHM_F64) launch_gaussian<double>((double*)h->vals, L.denseElems, seed, 1, st);
+    else launch_gaussian<float>((float*)h->vals, L.denseElems, seed, 1, st);
+    const Geometry& g = h->geo();
+    bool stdCaps = true;
+    for (int c : L.cap) stdCaps = stdCaps && c == std::max(h->maxRank, 1);
+    if (stdCaps) {
+        if (h->prec == DMHM_F64)
+            launch_gaussian<double>((double*)h->vals + L.denseElems, L.uElems + L.vElems, seed, 2, st);
+        else
+            launch_gaussian<float>((float*)h->vals + L.denseElems, L.uElems + L.vElems, seed, 2, st);
+    } else {
+        HIPCHECK(hipMemsetAsync((char*)h->vals + L.denseElems * h->esize(), 0, (L.uElems + L.vElems) * h->esize(), st));
+        for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+            const int b = g.lrBlocks[i];
+            const int64_t nu = (int64_t)g.tbox(b).size * h->maxRank, nv = (int64_t)g.sbox(b).size * h->maxRank;
+            if (h->prec == DMHM_F64) {
+                launch_gaussian<double>((double*)h->vals + L.uOff[i], nu, seed, 3 + 2 * i, st);
+                launch_gaussian<double>((double*)h->vals + L.vOff[i], nv, seed, 4 + 2 * i, st);
+            } else {
+                launch_gaussian<float>((float*)h->vals + L.uOff[i], nu, seed, 3 + 2 * i, st);
+                launch_gaussian<float>((float*)h->vals + L.vOff[i], nv, seed, 4 + 2 * i, st);
+            }
+        }
+    }
+    launch_fill_int(h->ranks, L.nLowRank(), h->maxRank, st);
+}
+
+template <typename T>
+static double estimate_two_norm_t(dmhm_hmat_s* h, double theta, double confidence, uint64_t seed)
+{
+    const Geometry& g = h->geo();
+    const int n = g.n;
+    const int k = (int)std::ceil(std::log(0.8 * std::sqrt((double)n) * std::pow(10.0, confidence)) /
+                                 std::log(theta));
+    g_lib.stageX.ensure(2 * (size_t)n * sizeof(T) + 64);
+    T* x = (T*)g_lib.stageX.ptr;
+    T* y = x + n;
+    hipStream_t st = g_lib.stream;
+    double* d = g_lib.dScalars;       // d[0] sumsq, d[1] log accumulator
+    launch_gaussian<T>(x, n, seed, 7, st);
+    launch_zero_double(d, 2, st);
+    launch_sumsq<T>(x, n, d, st);
+    launch_normalize<T>(x, n, d, nullptr, 0.0, st);
+    for (int i = 0; i < k; ++i) {
+        multiply_dense(h, false, 1.0, x, n, 1, 0.0, y, n);
+        launch_zero_double(d, 1, st);
+        launch_sumsq<T>(y, n, d, st);
+        launch_normalize<T>(y, n, d, d + 1, 1.0 / k, st);
+        std::swap(x, y);
+    }
+    double acc = 0;
+    HIPCHECK(hipMemcpyAsync(&acc, d + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return theta * std::exp(acc);
+}
+
+static double estimate_two_norm(dmhm_hmat_s* h, double theta, double confidence, uint64_t seed)
+{
+    if (theta <= 1) throw Error(DMHM_EINVAL, "Theta must be > 1");
+    if (confidence <= 0) throw Error(DMHM_EINVAL, "Confidence must be positive");
+    return h->prec == DMHM_F64 ? estimate_two_norm_t<double>(h, theta, confidence, seed)
+                               : estimate_two_norm_t<float>(h, theta, confidence, seed);
+}
+
+static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report* rep)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    if (o.theta <= 1) throw Error(DMHM_EINVAL, "Theta must be > 1");
+    if (o.confidence <= 0) throw Error(DMHM_EINVAL, "Confidence must be positive");
+    bool stop = false;
+    int numIt = o.numIterations;
+    if (numIt <= 0) { numIt = o.maxIterations > 0 ? o.maxIterations : 30; stop = true; }
+    uint64_t seed = o.hh.seed;
+    dmhm_hh_opts hh = o.hh;
+    std::unique_ptr<dmhm_hmat_s> Alow;
+    dmhm_hmat_s* Acur = A;
+    bool low = false;
+    if (o.mixed && A->prec == DMHM_F64) {
+        Alow.reset(copy_hmat(A, DMHM_F32));
+        Acur = Alow.get();
+        low = true;
+    }
+    const double est = estimate_two_norm(Acur, o.theta, o.confidence, seed++);
+    const double alpha = 2.0 / (est * est);
+    std::unique_ptr<dmhm_hmat_s> X(transpose_hmat(Acur));
+    scale_hmat(X.get(), alpha);
+    int it = 0, itLow = 0;
+    double e = -1;
+    for (int kk = 0; kk < numIt; ++kk) {
+        it = kk + 1;
+        if (low) ++itLow;
+        hh.seed = seed++;
+        std::unique_ptr<dmhm_hmat_s> Z(multiply_hmat(-1.0, X.get(), Acur, hh));
+        add_diag(Z.get(), 1.0);
+        bool done = false;
+        if (stop) {
+            e = estimate_two_norm(Z.get(), o.theta, o.confidence, seed++);
+            if (rep && kk < 128) rep->history[kk] = e;
+            if (std::fabs(e) < o.stopTol && !low) done = true;
+        }
+        add_diag(Z.get(), 1.0);
+        hh.seed = seed++;
+        std::unique_ptr<dmhm_hmat_s> Xn(multiply_hmat(1.0, Z.get(), X.get(), hh));
+        X = std::move(Xn);
+        if (done) break;
+        if (low && stop && std::fabs(e) < o.switchTol) {
+            // Switch to FP64 for the remaining iterations.
+            X.reset(copy_hmat(X.get(), DMHM_F64));
+            Acur = A;
+            low = false;
+        }
+    }
+    if (low) { X.reset(copy_hmat(X.get(), DMHM_F64)); }
+    HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    // Replace A's contents with X (CopyFrom, Invert-incl.hpp:195).
+    std::swap(A->lay, X->lay);
+    std::swap(A->vals, X->vals);
+    std::swap(A->ranks, X->ranks);
+    if (rep) {
+        rep->iterations = it;
+        rep->iterationsLowPrecision = itLow;
+        rep->residualEstimate = e;
+        rep->normEstimate = est;
+        rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (stop && !(std::fabs(e) < o.stopTol))
+        throw Error(DMHM_ENOCONV, "Schulz iteration did not reach the stop tolerance");
+}
+
+template <typename T>
+static void to_dense_t(dmhm_hmat_s* h, double* out, int ld)
+{
+    const Geometry& g = h->geo();
+    const Layout& L = *h->lay;
+    std::vector<T> v(L.totalElems());
+    std::vector<int> r(L.nLowRank());
+    HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    HIPCHECK(hipMemcpy(v.data(), h->vals, v.size() * sizeof(T), hipMemcpyDeviceToHost));
+    if (!r.empty()) HIPCHECK(hipMemcpy(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int j = 0; j < g.n; ++j)
+        for (int i = 0; i < g.n; ++i) out[i + (size_t)j * ld] = 0.0;
+    for (size_t i = 0; i < g.denseBlocks.size(); ++i) {
+        const int b = g.denseBlocks[i];
+        const Box& tb = g.tbox(b); const Box& sb = g.sbox(b);
+        const T* D = v.data() + L.denseOff[i];
+        for (int c = 0; c < sb.size; ++c)
+            for (int rr = 0; rr < tb.size; ++rr)
+                out[(tb.off + rr) + (size_t)(sb.off + c) * ld] = (double)D[rr + (size_t)c * tb.size];
+    }
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        const int b = g.lrBlocks[i];
+        const Box& tb = g.tbox(b); const Box& sb = g.sbox(b);
+        const T* U = v.data() + L.uOff[i];
+        const T* V = v.data() + L.vOff[i];
+        for (int c = 0; c < sb.size; ++c)
+            for (int rr = 0; rr < tb.size; ++rr) {
+                double s = 0;
+                for (int p = 0; p < r[i]; ++p) s += (double)U[rr + (size_t)p * tb.size] * (double)V[c + (size_t)p * sb.size];
+                out[(tb.off + rr) + (size_t)(sb.off + c) * ld] = s;
+            }
+    }
+}
+
+static dmhm_hmat_s* create_from_csr(int dim, const int* sizes, int L, int maxRank, bool strong, int prec,
+                                    const int* rowptr, const int* colind, const double* vals)
+{
+    auto gc = get_geo(dim, sizes, L, strong);
+    const Geometry& g = gc->geo;
+    // Import ranks (number of nonzeros inside each admissible block).
+    // Map every nonzero to its leaf-level (row box, col box), then walk up to
+    // find the admissible ancestor block.
+    const int Lf = g.numLevels - 1;
+    std::vector<int> leafBoxOfRow(g.n);
+    for (int b = 0; b < (int)g.boxes[Lf].size(); ++b) {
+        const Box& B = g.boxes[Lf][b];
+        for (int i = 0; i < B.size; ++i) leafBoxOfRow[B.off + i] = b;
+    }
+    auto leafOf = [&](int row, int col, int& blockId) {
+        int tb = leafBoxOfRow[row], sb = leafBoxOfRow[col];
+        // ancestors at each level
+        std::vector<int> ta(g.numLevels), sa(g.numLevels);
+        ta[Lf] = tb; sa[Lf] = sb;
+        for (int l = Lf; l > 0; --l) { ta[l - 1] = g.boxes[l][ta[l]].parent; sa[l - 1] = g.boxes[l][sa[l]].parent; }
+        for (int l = 0; l <= Lf; ++l) {
+            const int id = g.find(l, ta[l], sa[l]);
+            if (id >= 0 && g.blocks[id].type != BT_NODE) { blockId = id; return; }
+        }
+        throw Error(DMHM_EINVAL, "internal: no leaf block for a nonzero");
+    };
+    const int nnz = rowptr[g.n];
+    std::vector<int> nzBlock(nnz);
+    std::vector<int> importRank(g.lrBlocks.size(), 0);
+    for (int i = 0; i < g.n; ++i)
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            if (colind[k] < 0 || colind[k] >= g.n) throw Error(DMHM_EINVAL, "column index out of range");
+            int id; leafOf(i, colind[k], id);
+            nzBlock[k] = id;
+            if (g.blocks[id].type == BT_LOWRANK) importRank[g.blocks[id].lrIndex]++;
+        }
+    bool anyRank = false;
+    std::vector<int> caps(g.lrBlocks.size());
+    bool stdCaps = true;
+    for (size_t i = 0; i < caps.size(); ++i) {
+        caps[i] = std::max(std::max(maxRank, 1), importRank[i]);
+        stdCaps = stdCaps && caps[i] == std::max(maxRank, 1);
+        anyRank = anyRank || importRank[i] > 0;
+    }
+    std::shared_ptr<Layout> lay;
+    if (stdCaps) lay = std_layout(*gc, maxRank);
+    else {
+        std::lock_guard<std::mutex> lk(g_lib.mu);
+        lay = std::make_shared<Layout>(Layout::make(g, caps, g_lib.nextLayoutId++));
+    }
+    std::unique_ptr<dmhm_hmat_s> h(alloc_hmat(gc, lay, prec, maxRank));
+    // Host staging of the dense pool (and low-rank pools when ranks > 0).
+    const int64_t hostElems = anyRank ? lay->totalElems() : lay->denseElems;
+    std::vector<double> host(hostElems, 0.0);
+    std::vector<int> fill(g.lrBlocks.size(), 0);
+    for (int i = 0; i < g.n; ++i)
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            const int id = nzBlock[k];
+            const Block& B = g.blocks[id];
+            const Box& tb = g.tbox(id); const Box& sb = g.sbox(id);
+            const int r = i - tb.off, c = colind[k] - sb.off;
+            if (B.type == BT_DENSE) {
+                host[lay->denseOff[B.denseIndex] + r + (int64_t)c * tb.size] = vals[k];
+            } else {
+                // ConvertSubmatrix (LowRank): U = unit column, V = value column.
+                const int li = B.lrIndex, p = fill[li]++;
+                host[lay->uOff[li] + r + (int64_t)p * tb.size] = 1.0;
+                host[lay->vOff[li] + c + (int64_t)p * sb.size] = vals[k];
+            }
+        }
+    if (prec == DMHM_F64) {
+        HIPCHECK(hipMemcpy(h->vals, host.data(), hostElems * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> hf(host.begin(), host.end());
+        HIPCHECK(hipMemcpy(h->vals, hf.data(), hostElems * 4, hipMemcpyHostToDevice));
+    }
+    if (!importRank.empty())
+        HIPCHECK(hipMemcpy(h->ranks, importRank.data(), importRank.size() * sizeof(int), hipMemcpyHostToDevice));
+    return h.release();
+}
+
+static void matvec_bytes(dmhm_hmat_s* h, int k, double* bytes, double* flops)
+{
+    const Geometry& g = h->geo();
+    const Layout& L = *h->lay;
+    std::vector<int> r(L.nLowRank());
+    HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    if (!r.empty()) HIPCHECK(hipMemcpy(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
+    double entries = 0;
+    for (int b : g.denseBlocks) entries += (double)g.tbox(b).size * g.sbox(b).size;
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        const int b = g.lrBlocks[i];
+        entries += (double)(g.tbox(b).size + g.sbox(b).size) * r[i];
+    }
+    const double es = (double)h->esize();
+    if (bytes) *bytes = es * entries + 2.0 * es * g.n * k;
+    if (flops) *flops = 2.0 * k * entries;
+}
+
+static int guard(const std::function<void()>& fn)
+{
+    try {
+        fn();
+        return DMHM_OK;
+    } catch (const Error& e) {
+        g_lastError = e.what();
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        g_lastError = e.what();
+        return DMHM_EINVAL;
+    } catch (const std::logic_error& e) {
+        g_lastError = e.what();
+        return DMHM_EINVAL;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return DMHM_EHIP;
+    }
+}
+
+} // namespace dmhm
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int dmhm_init(int device)
+{
+    return guard([&] {
+        if (g_lib.ready) return;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+            throw Error(DMHM_ENODEV, "no HIP device available");
+        if (device < 0 || device >= count) throw Error(DMHM_EINVAL, "device index out of range");
+        HIPCHECK(hipSetDevice(device));
+        HIPCHECK(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
+        HIPCHECK(hipMalloc((void**)&g_lib.dScalars, 16 * sizeof(double)));
+        g_lib.device = device;
+        g_lib.ready = true;
+    });
+}
+
+int dmhm_finalize(void)
+{
+    return guard([&] {
+        if (!g_lib.ready) return;
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        g_lib.ws.release(); g_lib.stageX.release(); g_lib.stageY.release();
+        (void)hipFree(g_lib.dScalars);
+        (void)hipStreamDestroy(g_lib.stream);
+        g_lib.ready = false;
+    });
+}
+
+const char* dmhm_last_error(void) { return g_lastError.c_str(); }
+
+int dmhm_synchronize(void)
+{
+    return guard([&] { require_ready(); HIPCHECK(hipStreamSynchronize(g_lib.stream)); });
+}
+
+int dmhm_build_natural_to_hierarchical_map(int dim, const int* sizes, int numLevels, int* map)
+{
+    return guard([&] {
+        if (!sizes || !map) throw Error(DMHM_EINVAL, "null pointer");
+        Geometry g(dim, sizes, numLevels, true);
+        auto m = g.naturalToHierarchical();
+        std::copy(m.begin(), m.end(), map);
+    });
+}
+
+int dmhm_hmat_create_from_csr(int dim, const int* sizes, int numLevels, int maxRank, int strong,
+                              int precision, const int* rowOffsets, const int* columnIndices,
+                              const double* nonzeros, dmhm_hmat* out)
+{
+    return guard([&] {
+        require_ready();
+        if (!sizes || !rowOffsets || !columnIndices || !nonzeros || !out) throw Error(DMHM_EINVAL, "null pointer");
+        if (precision != DMHM_F64 && precision != DMHM_F32) throw Error(DMHM_EINVAL, "precision must be 8 or 4");
+        if (maxRank < 0 || maxRank > 64) throw Error(DMHM_EINVAL, "maxRank must be in [0, 64]");
+        *out = create_from_csr(dim, sizes, numLevels, maxRank, strong != 0, precision, rowOffsets, columnIndices,
+                               nonzeros);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_create_zero(int dim, const int* sizes, int numLevels, int maxRank, int strong, int precision,
+                          dmhm_hmat* out)
+{
+    return guard([&] {
+        require_ready();
+        if (!sizes || !out) throw Error(DMHM_EINVAL, "null pointer");
+        if (precision != DMHM_F64 && precision != DMHM_F32) throw Error(DMHM_EINVAL, "precision must be 8 or 4");
+        if (maxRank < 0 || maxRank > 64) throw Error(DMHM_EINVAL, "maxRank must be in [0, 64]");
+        auto gc = get_geo(dim, sizes, numLevels, strong != 0);
+        *out = alloc_hmat(gc, std_layout(*gc, maxRank), precision, maxRank);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_destroy(dmhm_hmat h)
+{
+    return guard([&] {
+        if (!h) return;
+        if (g_lib.ready) HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        delete h;
+    });
+}
+
+int dmhm_hmat_info(dmhm_hmat h, dmhm_info* info)
+{
+    return guard([&] {
+        if (!h || !info) throw Error(DMHM_EINVAL, "null pointer");
+        const Geometry& g = h->geo();
+        info->dim = g.dim; info->n = g.n; info->numLevels = g.numLevels; info->maxRank = h->maxRank;
+        info->strong = g.strong; info->precision = h->prec;
+        info->numDense = (long long)g.denseBlocks.size();
+        info->numLowRank = (long long)g.lrBlocks.size();
+        info->storageBytes = (long long)(h->lay->totalElems() * h->esize());
+        std::vector<int> r(h->lay->nLowRank());
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        if (!r.empty()) HIPCHECK(hipMemcpy(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
+        long long s = 0; int mx = 0;
+        for (int v : r) { s += v; mx = std::max(mx, v); }
+        info->sumRank = s; info->maxBlockRank = mx;
+    });
+}
+
+int dmhm_hmat_block_counts(dmhm_hmat h, long long* dense, long long* lowRank, int maxLevels)
+{
+    return guard([&] {
+        if (!h || !dense || !lowRank) throw Error(DMHM_EINVAL, "null pointer");
+        std::vector<long> d, f;
+        h->geo().counts(d, f);
+        for (int l = 0; l < maxLevels; ++l) {
+            dense[l] = l < (int)d.size() ? d[l] : 0;
+            lowRank[l] = l < (int)f.size() ? f[l] : 0;
+        }
+    });
+}
+
+int dmhm_hmat_set_to_random(dmhm_hmat h, uint64_t seed)
+{
+    return guard([&] {
+        require_ready();
+        if (!h) throw Error(DMHM_EINVAL, "null handle");
+        set_to_random(h, seed);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_multiply_dense(dmhm_hmat h, char op, double alpha, const void* X, int ldx, int k, double beta,
+                             void* Y, int ldy, int memory)
+{
+    return guard([&] {
+        require_ready();
+        if (!h || !X || !Y) throw Error(DMHM_EINVAL, "null pointer");
+        const int n = h->geo().n;
+        if (k < 0 || ldx < n || ldy < n) throw Error(DMHM_EINVAL, "bad leading dimension or column count");
+        if (k == 0) return;
+        const bool trans = (op == 'T' || op == 't' || op == 'C' || op == 'c');
+        if (!trans && op != 'N' && op != 'n') throw Error(DMHM_EINVAL, "op must be N, T or C");
+        if (memory == DMHM_DEVICE) {
+            multiply_dense(h, trans, alpha, X, ldx, k, beta, Y, ldy);
+        } else {
+            const size_t es = h->esize();
+            const size_t bx = (size_t)ldx * k * es, by = (size_t)ldy * k * es;
+            g_lib.stageX.ensure(bx);
+            g_lib.stageY.ensure(by);
+            HIPCHECK(hipMemcpyAsync(g_lib.stageX.ptr, X, bx, hipMemcpyHostToDevice, g_lib.stream));
+            if (beta != 0.0) HIPCHECK(hipMemcpyAsync(g_lib.stageY.ptr, Y, by, hipMemcpyHostToDevice, g_lib.stream));
+            multiply_dense(h, trans, alpha, g_lib.stageX.ptr, ldx, k, beta, g_lib.stageY.ptr, ldy);
+            HIPCHECK(hipMemcpyAsync(Y, g_lib.stageY.ptr, by, hipMemcpyDeviceToHost, g_lib.stream));
+        }
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_multiply_hmat(double alpha, dmhm_hmat A, dmhm_hmat B, dmhm_hmat* C, const dmhm_hh_opts* opts)
+{
+    return guard([&] {
+        require_ready();
+        if (!A || !B || !C) throw Error(DMHM_EINVAL, "null pointer");
+        dmhm_hh_opts o{4, 0, 1e-10, 1};
+        if (opts) o = *opts;
+        *C = multiply_hmat(alpha, A, B, o);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_copy(dmhm_hmat src, dmhm_hmat* dst)
+{
+    return guard([&] {
+        require_ready();
+        if (!src || !dst) throw Error(DMHM_EINVAL, "null pointer");
+        *dst = copy_hmat(src, src->prec);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_convert(dmhm_hmat src, int precision, dmhm_hmat* dst)
+{
+    return guard([&] {
+        require_ready();
+        if (!src || !dst) throw Error(DMHM_EINVAL, "null pointer");
+        if (precision != DMHM_F64 && precision != DMHM_F32) throw Error(DMHM_EINVAL, "precision must be 8 or 4");
+        *dst = copy_hmat(src, precision);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_transpose(dmhm_hmat src, dmhm_hmat* dst)
+{
+    return guard([&] {
+        require_ready();
+        if (!src || !dst) throw Error(DMHM_EINVAL, "null pointer");
+        *dst = transpose_hmat(src);
+    });
+}
+
+int dmhm_hmat_scale(dmhm_hmat h, double alpha)
+{
+    return guard([&] {
+        require_ready();
+        if (!h) throw Error(DMHM_EINVAL, "null handle");
+        scale_hmat(h, alpha);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_set_to_identity(dmhm_hmat h)
+{
+    return guard([&] {
+        require_ready();
+        if (!h) throw Error(DMHM_EINVAL, "null handle");
+        scale_hmat(h, 0.0);
+        add_diag(h, 1.0);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_add_constant_to_diagonal(dmhm_hmat h, double alpha)
+{
+    return guard([&] {
+        require_ready();
+        if (!h) throw Error(DMHM_EINVAL, "null handle");
+        add_diag(h, alpha);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_estimate_two_norm(dmhm_hmat h, double theta, double confidence, uint64_t seed, double* estimate)
+{
+    return guard([&] {
+        require_ready();
+        if (!h || !estimate) throw Error(DMHM_EINVAL, "null pointer");
+        *estimate = estimate_two_norm(h, theta, confidence, seed);
+    });
+}
+
+int dmhm_hmat_schulz_invert(dmhm_hmat h, const dmhm_schulz_opts* opts, dmhm_schulz_report* report)
+{
+    return guard([&] {
+        require_ready();
+        if (!h) throw Error(DMHM_EINVAL, "null handle");
+        if (h->geo().n == 0) throw Error(DMHM_EINVAL, "empty matrix");
+        dmhm_schulz_opts o{};
+        if (opts) o = *opts;
+        else { o.numIterations = -1; o.maxIterations = 30; o.theta = 1.5; o.confidence = 6; o.stopTol = 1e-4;
+               o.hh = {4, 0, 1e-10, 1}; }
+        if (report) std::memset(report, 0, sizeof(*report));
+        schulz(h, o, report);
+    });
+}
+
+int dmhm_hmat_to_dense(dmhm_hmat h, double* out, int ld)
+{
+    return guard([&] {
+        require_ready();
+        if (!h || !out) throw Error(DMHM_EINVAL, "null pointer");
+        if (ld < h->geo().n) throw Error(DMHM_EINVAL, "ld too small");
+        if (h->prec == DMHM_F64) to_dense_t<double>(h, out, ld);
+        else to_dense_t<float>(h, out, ld);
+    });
+}
+
+int dmhm_hmat_ranks(dmhm_hmat h, int* ranks, int count)
+{
+    return guard([&] {
+        require_ready();
+        if (!h || !ranks) throw Error(DMHM_EINVAL, "null pointer");
+        const int n = std::min(count, h->lay->nLowRank());
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        if (n > 0) HIPCHECK(hipMemcpy(ranks, h->ranks, n * sizeof(int), hipMemcpyDeviceToHost));
+    });
+}
+
+int dmhm_hmat_matvec_bytes(dmhm_hmat h, int k, double* bytes, double* flops)
+{
+    return guard([&] {
+        require_ready();
+        if (!h) throw Error(DMHM_EINVAL, "null handle");
+        matvec_bytes(h, k, bytes, flops);
+    });
+}
+
+} // extern "C"
+
+// Plan upload/release (kept here so plan.cpp stays host-only C++).
+namespace dmhm {
+void Plan::upload()
+{
+    for (auto& ph : phases) {
+        if (!ph.tiles.empty()) {
+            HIPCHECK(hipMalloc(&ph.dTiles, ph.tiles.size() * sizeof(Tile)));
+            HIPCHECK(hipMemcpy(ph.dTiles, ph.tiles.data(), ph.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        }
+        if (!ph.contribs.empty()) {
+            HIPCHECK(hipMalloc(&ph.dContribs, ph.contribs.size() * sizeof(Contrib)));
+            HIPCHECK(hipMemcpy(ph.dContribs, ph.contribs.data(), ph.contribs.size() * sizeof(Contrib),
+                               hipMemcpyHostToDevice));
+        }
+        const void* src = nullptr; size_t bytes = 0;
+        if (ph.kind == PH_QR && !ph.qr.empty()) { src = ph.qr.data(); bytes = ph.qr.size() * sizeof(QrTask); }
+        if (ph.kind == PH_SVD && !ph.svd.empty()) { src = ph.svd.data(); bytes = ph.svd.size() * sizeof(SvdTask); }
+        if (ph.kind == PH_RNG && !ph.rng.empty()) { src = ph.rng.data(); bytes = ph.rng.size() * sizeof(RngTask); }
+        if (src) {
+            HIPCHECK(hipMalloc(&ph.dTasks, bytes));
+            HIPCHECK(hipMemcpy(ph.dTasks, src, bytes, hipMemcpyHostToDevice));
+        }
+        // Host copies of the big arrays are no longer needed.
+        std::vector<Contrib>().swap(ph.contribs);
+    }
+}
+
+void Plan::release()
+{
+    for (auto& ph : phases) {
+        if (ph.dTiles) (void)hipFree(ph.dTiles);
+        if (ph.dContribs) (void)hipFree(ph.dContribs);
+        if (ph.dTasks) (void)hipFree(ph.dTasks);
+        ph.dTiles = ph.dContribs = ph.dTasks = nullptr;
+    }
+}
+} // namespace dmhm

@@ -1,0 +1,622 @@
+// gfx950 kernels for the H-matrix path. Wave64 throughout.
+//
+//  tile_kernel   grouped output-owned small GEMMs (the H*X, sub-H products,
+//                projections, dense-leaf accumulation and factor formation)
+//  qr_kernel     batched Householder QR of tall-skinny panels, Q formed in place
+//                (geqrf + orgqr semantics, one workgroup per panel)
+//  svd_kernel    batched one-sided Jacobi SVD of k x k factors (k <= 64) held
+//                in LDS, FP64, with the rank decision
+//  rng_kernel    Gaussian sketch panels (counter-based, order independent)
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include "kernels.hpp"
+
+namespace dmhm {
+
+#define HIP_LAUNCH_CHECK()                                                        \
+    do {                                                                          \
+        hipError_t e_ = hipGetLastError();                                        \
+        if (e_ != hipSuccess)                                                     \
+            throw std::runtime_error(std::string("HIP launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Tile GEMM engine.
+// Workgroup = 256 threads = 4 waves. Thread (i = tid & 63, g = tid >> 6) owns
+// output rows i and columns j = g + 4*kk, kk < KPT. L (m x q) and R (q x k)
+// are staged through LDS in chunks of 64 along q; R is pre-scaled by alpha.
+// ---------------------------------------------------------------------------
+template <typename T, int KPT>
+__global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restrict__ tiles,
+                                                   const Contrib* __restrict__ cs, T alpha, T beta)
+{
+    __shared__ T Ls[64][65];
+    __shared__ T Rs[64][65];
+    const Tile tl = tiles[blockIdx.x];
+    int m = tl.m, k = tl.k;
+    if (tl.mrank >= 0) m = min(m, max(0, b.rank[tl.mslot][tl.mrank] - tl.moff));
+    if (tl.krank >= 0) k = min(k, max(0, b.rank[tl.kslot][tl.krank] - tl.koff));
+    if (m <= 0 || k <= 0) return;
+    const int tid = threadIdx.x, i = tid & 63, g = tid >> 6;
+    T acc[KPT];
+#pragma unroll
+    for (int kk = 0; kk < KPT; ++kk) acc[kk] = T(0);
+
+    for (int c = tl.cb; c < tl.ce; ++c) {
+        const Contrib C = cs[c];
+        int q = C.q;
+        if (C.qrank >= 0) q = min(q, b.rank[C.qslot][C.qrank]);
+        if (q <= 0) continue;
+        const T* __restrict__ L = reinterpret_cast<const T*>(b.val[C.lslot]) + C.loff;
+        const T* __restrict__ R = reinterpret_cast<const T*>(b.val[C.rslot]) + C.roff;
+        const T a = (C.flags & CF_RUNTIME_ALPHA) ? alpha * T(C.alpha) : T(C.alpha);
+        const bool tL = C.flags & CF_TRANS_L, tR = C.flags & CF_TRANS_R;
+        for (int p0 = 0; p0 < q; p0 += 64) {
+            const int qc = min(64, q - p0);
+            if (!tL) {
+                for (int e = tid; e < 64 * qc; e += 256) {
+                    const int ii = e & 63, pp = e >> 6;
+                    Ls[pp][ii] = (ii < m) ? L[ii + (int64_t)(p0 + pp) * C.ldl] : T(0);
+                }
+            } else {
+                for (int e = tid; e < 64 * 64; e += 256) {
+                    const int pp = e & 63, ii = e >> 6;
+                    if (pp < qc) Ls[pp][ii] = (ii < m) ? L[(p0 + pp) + (int64_t)ii * C.ldl] : T(0);
+                }
+            }
+            if (!tR) {
+                for (int e = tid; e < 64 * 64; e += 256) {
+                    const int pp = e & 63, jj = e >> 6;
+                    if (pp < qc) Rs[pp][jj] = (jj < k) ? a * R[(p0 + pp) + (int64_t)jj * C.ldr] : T(0);
+                }
+            } else {
+                for (int e = tid; e < 64 * qc; e += 256) {
+                    const int jj = e & 63, pp = e >> 6;
+                    Rs[pp][jj] = (jj < k) ? a * R[jj + (int64_t)(p0 + pp) * C.ldr] : T(0);
+                }
+            }
+            __syncthreads();
+            for (int p = 0; p < qc; ++p) {
+                const T l = Ls[p][i];
+#pragma unroll
+                for (int kk = 0; kk < KPT; ++kk) acc[kk] += l * Rs[p][g + 4 * kk];
+            }
+            __syncthreads();
+        }
+    }
+    if (i < m) {
+        T* out = reinterpret_cast<T*>(b.val[tl.slot]) + tl.off;
+        const bool useBeta = (tl.flags & TF_RUNTIME_BETA) && beta != T(0);
+        const bool accum = (tl.flags & TF_ACCUMULATE);
+#pragma unroll
+        for (int kk = 0; kk < KPT; ++kk) {
+            const int j = g + 4 * kk;
+            if (j < k) {
+                T* o = out + i + (int64_t)j * tl.ld;
+                if (useBeta) *o = beta * *o + acc[kk];
+                else if (accum) *o += acc[kk];
+                else *o = acc[kk];
+            }
+        }
+    }
+}
+
+template <typename T>
+void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s)
+{
+    const int n = (int)ph.tiles.size();
+    if (n == 0) return;
+    const Tile* t = static_cast<const Tile*>(ph.dTiles);
+    const Contrib* c = static_cast<const Contrib*>(ph.dContribs);
+    const int kmax = ph.maxKTile;
+    if (kmax <= 4) tile_kernel<T, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else if (kmax <= 8) tile_kernel<T, 2><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else if (kmax <= 16) tile_kernel<T, 4><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else if (kmax <= 32) tile_kernel<T, 8><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else tile_kernel<T, 16><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    HIP_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Batched Householder QR (one workgroup per panel, FP64 arithmetic).
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// dots[c] = sum_{i > j} A(i, j) * A(i, c) for c in [c0, k)
+template <typename T>
+__device__ void column_dots(const T* A, int m, int j, int c0, int k, double* dots,
+                            double (*red)[16])
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int cb = c0; cb < k; cb += 16) {
+        double p[16];
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) p[cc] = 0.0;
+        for (int i = j + 1 + tid; i < m; i += 256) {
+            const double x = (double)A[i + (int64_t)j * m];
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc)
+                if (cb + cc < k) p[cc] += x * (double)A[i + (int64_t)(cb + cc) * m];
+        }
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) {
+            const double v = wave_sum(p[cc]);
+            if (lane == 0) red[wave][cc] = v;
+        }
+        __syncthreads();
+        if (tid < 16 && cb + tid < k) dots[cb + tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) qr_kernel(Bases b, const QrTask* __restrict__ tasks)
+{
+    __shared__ double red[4][16];
+    __shared__ double dots[64];
+    __shared__ double wv[64];
+    __shared__ double tau_s[64];
+    __shared__ double scale_s;
+    const QrTask tk = tasks[blockIdx.x];
+    T* A = reinterpret_cast<T*>(b.val[tk.slot]) + tk.off;
+    const int m = tk.m, k = tk.k, kk = min(m, k);
+    const int tid = threadIdx.x;
+    if (m <= 0) return;
+    // Factor.
+    for (int j = 0; j < kk; ++j) {
+        column_dots(A, m, j, j, k, dots, red);
+        if (tid == 0) {
+            const double alpha = (double)A[j + (int64_t)j * m];
+            const double xn2 = dots[j];
+            double tau = 0.0, scale = 0.0, beta = alpha;
+            if (xn2 > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                tau = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            tau_s[j] = tau; scale_s = scale;
+            A[j + (int64_t)j * m] = (T)beta;
+        }
+        __syncthreads();
+        const double scale = scale_s, tau = tau_s[j];
+        if (tid > j && tid < k) {
+            const int c = tid;
+            const double w = tau * ((double)A[j + (int64_t)c * m] + scale * dots[c]);
+            wv[c] = w;
+            A[j + (int64_t)c * m] = (T)((double)A[j + (int64_t)c * m] - w);
+        }
+        __syncthreads();
+        for (int i = j + 1 + tid; i < m; i += 256) {
+            const double v = (double)A[i + (int64_t)j * m] * scale;
+            for (int c = j + 1; c < k; ++c)
+                A[i + (int64_t)c * m] = (T)((double)A[i + (int64_t)c * m] - wv[c] * v);
+            A[i + (int64_t)j * m] = (T)v;
+        }
+        __syncthreads();
+    }
+    // R (k x k, upper triangle; rows >= kk zero).
+    if (tk.roff >= 0) {
+        T* R = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.roff;
+        for (int e = tid; e < k * k; e += 256) {
+            const int r = e % k, c = e / k;
+            R[e] = (r <= c && r < kk) ? A[r + (int64_t)c * m] : T(0);
+        }
+    }
+    __syncthreads();
+    // Form Q in place (orgqr); columns >= kk are zero.
+    for (int c = kk + (tid >> 6); c < k; c += 4)
+        for (int i = tid & 63; i < m; i += 64) A[i + (int64_t)c * m] = T(0);
+    __syncthreads();
+    for (int j = kk - 1; j >= 0; --j) {
+        const double tau = tau_s[j];
+        if (j + 1 < k) column_dots(A, m, j, j + 1, k, dots, red);
+        if (tid > j && tid < k) {
+            const int c = tid;
+            const double w = tau * ((double)A[j + (int64_t)c * m] + dots[c]);
+            wv[c] = w;
+            A[j + (int64_t)c * m] = (T)((double)A[j + (int64_t)c * m] - w);
+        }
+        __syncthreads();
+        for (int i = j + 1 + tid; i < m; i += 256) {
+            const double v = (double)A[i + (int64_t)j * m];
+            for (int c = j + 1; c < k; ++c)
+                A[i + (int64_t)c * m] = (T)((double)A[i + (int64_t)c * m] - wv[c] * v);
+            A[i + (int64_t)j * m] = (T)(-tau * v);
+        }
+        for (int i = tid; i < j; i += 256) A[i + (int64_t)j * m] = T(0);
+        if (tid == 0) A[j + (int64_t)j * m] = (T)(1.0 - tau);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+void launch_qr_phase(const Phase& ph, const Bases& b, hipStream_t s)
+{
+    const int n = (int)ph.qr.size();
+    if (n == 0) return;
+    qr_kernel<T><<<n, 256, 0, s>>>(b, static_cast<const QrTask*>(ph.dTasks));
+    HIP_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Batched one-sided Jacobi SVD of k x k (k <= 64), FP64 in LDS.
+// R = U S V^T; outputs G1 = V S (first `keep` columns, rest zero) and
+// G2 = U (same), both k x cap, and the kept rank.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) svd_kernel(Bases b, const SvdTask* __restrict__ tasks,
+                                                  int tolMode, double relTol)
+{
+    __shared__ double As[64][65];
+    __shared__ double Vs[64][65];
+    __shared__ double sig[64];
+    __shared__ int order[64];
+    __shared__ int rotated;
+    __shared__ int keep_s;
+    const SvdTask tk = tasks[blockIdx.x];
+    const int k = tk.k, tid = threadIdx.x;
+    T* ws = reinterpret_cast<T*>(b.val[SLOT_WS]);
+    for (int e = tid; e < k * k; e += 256) {
+        const int r = e % k, c = e / k;
+        As[c][r] = (double)ws[tk.roff + e];
+        Vs[c][r] = (r == c) ? 1.0 : 0.0;
+    }
+    const int kp = k + (k & 1);
+    const int pi = tid >> 3, sub = tid & 7;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        __syncthreads();
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int round = 0; round < kp - 1; ++round) {
+            if (pi < kp / 2) {
+                auto pos = [&](int x) { return x == 0 ? 0 : 1 + ((x - 1 + round) % (kp - 1)); };
+                const int a0 = pos(pi), a1 = pos(kp - 1 - pi);
+                const int p = min(a0, a1), q = max(a0, a1);
+                double al = 0, be = 0, ga = 0;
+                if (q < k)
+                    for (int r = sub; r < k; r += 8) {
+                        const double x = As[p][r], y = As[q][r];
+                        al += x * x; be += y * y; ga += x * y;
+                    }
+#pragma unroll
+                for (int off = 4; off >= 1; off >>= 1) {
+                    al += __shfl_xor(al, off, 64);
+                    be += __shfl_xor(be, off, 64);
+                    ga += __shfl_xor(ga, off, 64);
+                }
+                if (q < k && ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
+                    const double zeta = (be - al) / (2.0 * ga);
+                    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                    for (int r = sub; r < k; r += 8) {
+                        const double x = As[p][r], y = As[q][r];
+                        As[p][r] = cs * x - sn * y; As[q][r] = sn * x + cs * y;
+                        const double vx = Vs[p][r], vy = Vs[q][r];
+                        Vs[p][r] = cs * vx - sn * vy; Vs[q][r] = sn * vx + cs * vy;
+                    }
+                    if (sub == 0) rotated = 1;
+                }
+            }
+            __syncthreads();
+        }
+        const int rot = rotated;
+        if (!rot) break;
+    }
+    __syncthreads();
+    if (tid < k) {
+        double s = 0;
+        for (int r = 0; r < k; ++r) s += As[tid][r] * As[tid][r];
+        sig[tid] = sqrt(s);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int c = 0; c < k; ++c) order[c] = c;
+        for (int a = 0; a < k; ++a) {          // selection sort, descending
+            int best = a;
+            for (int c = a + 1; c < k; ++c)
+                if (sig[order[c]] > sig[order[best]]) best = c;
+            const int tmp = order[a]; order[a] = order[best]; order[best] = tmp;
+        }
+        int keep = tk.maxKeep;
+        if (tolMode == 1) {
+            const double s0 = sig[order[0]];
+            if (s0 < relTol * relTol) keep = min(1, keep);
+            else {
+                int r = 0;
+                while (r < keep && sig[order[r]] > relTol * s0) ++r;
+                keep = r;
+            }
+        }
+        keep_s = keep;
+        b.rank[SLOT_C][tk.rankIndex] = keep;
+    }
+    __syncthreads();
+    const int keep = keep_s;
+    for (int e = tid; e < k * tk.cap; e += 256) {
+        const int r = e % k, j = e / k;
+        double g1 = 0.0, g2 = 0.0;
+        if (j < keep) {
+            const int c = order[j];
+            g1 = Vs[c][r] * sig[c];
+            g2 = sig[c] > 0 ? As[c][r] / sig[c] : 0.0;
+        }
+        ws[tk.g1off + e] = (T)g1;
+        ws[tk.g2off + e] = (T)g2;
+    }
+}
+
+template <typename T>
+void launch_svd_phase(const Phase& ph, const Bases& b, TruncPolicy pol, hipStream_t s)
+{
+    const int n = (int)ph.svd.size();
+    if (n == 0) return;
+    svd_kernel<T><<<n, 256, 0, s>>>(b, static_cast<const SvdTask*>(ph.dTasks), pol.tolMode, pol.relTol);
+    HIP_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based Gaussian generator (splitmix64 + Box-Muller).
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t mix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+__device__ inline double gauss_at(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    const uint64_t h = mix64(seed ^ mix64(stream * 0xD1B54A32D192ED03ULL + mix64(idx)));
+    const double u1 = ((double)(h >> 32) + 1.0) / 4294967296.0;
+    const double u2 = (double)(h & 0xFFFFFFFFULL) / 4294967296.0;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) rng_kernel(Bases b, const RngTask* __restrict__ tasks,
+                                                  uint64_t seed)
+{
+    const RngTask tk = tasks[blockIdx.x];
+    T* x = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.off;
+    const int64_t n = (int64_t)tk.m * tk.k;
+    for (int64_t e = threadIdx.x; e < n; e += 256) x[e] = (T)gauss_at(seed, tk.stream, (uint64_t)e);
+}
+
+template <typename T>
+void launch_rng_phase(const Phase& ph, const Bases& b, uint64_t seed, hipStream_t s)
+{
+    const int n = (int)ph.rng.size();
+    if (n == 0) return;
+    rng_kernel<T><<<n, 256, 0, s>>>(b, static_cast<const RngTask*>(ph.dTasks), seed);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void gaussian_kernel(T* x, int64_t n, uint64_t seed, uint64_t stream)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        x[e] = (T)gauss_at(seed, stream, (uint64_t)e);
+}
+
+static int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
+
+template <typename T> void launch_gaussian(T* x, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s)
+{
+    if (n <= 0) return;
+    gaussian_kernel<T><<<grid_for(n), 256, 0, s>>>(x, n, seed, stream);
+    HIP_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise and per-leaf helpers.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void scale_kernel(T* x, int64_t n, T a)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        x[e] = a * x[e];
+}
+template <typename T> void launch_scale(T* x, int64_t n, T alpha, hipStream_t s)
+{
+    if (n <= 0) return;
+    scale_kernel<T><<<grid_for(n), 256, 0, s>>>(x, n, alpha);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void fill_kernel(T* x, int64_t n, T v)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        x[e] = v;
+}
+template <typename T> void launch_fill(T* x, int64_t n, T v, hipStream_t s)
+{
+    if (n <= 0) return;
+    fill_kernel<T><<<grid_for(n), 256, 0, s>>>(x, n, v);
+    HIP_LAUNCH_CHECK();
+}
+void launch_fill_int(int* x, int64_t n, int v, hipStream_t s)
+{
+    if (n <= 0) return;
+    fill_kernel<int><<<grid_for(n), 256, 0, s>>>(x, n, v);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename Ts, typename Td>
+__global__ void convert_kernel(const Ts* x, Td* y, int64_t n)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        y[e] = (Td)x[e];
+}
+template <typename Ts, typename Td> void launch_convert(const Ts* x, Td* y, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    convert_kernel<Ts, Td><<<grid_for(n), 256, 0, s>>>(x, y, n);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void add_diag_kernel(T* vals, const int64_t* offs, const int* sizes, int count, T a)
+{
+    const int blk = blockIdx.x;
+    if (blk >= count) return;
+    T* D = vals + offs[blk];
+    const int n = sizes[blk];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) D[i + (int64_t)i * n] += a;
+}
+template <typename T> void launch_add_diag(T* vals, const int64_t* offs, const int* sizes, int count,
+                                           T alpha, hipStream_t s)
+{
+    if (count <= 0) return;
+    add_diag_kernel<T><<<count, 64, 0, s>>>(vals, offs, sizes, count, alpha);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) transpose_dense_kernel(const T* src, T* dst, const int64_t* srcOff,
+                                                              const int64_t* dstOff, const int* mm,
+                                                              const int* nn)
+{
+    __shared__ T tile[64][65];
+    const int blk = blockIdx.x;
+    const T* S = src + srcOff[blk];
+    T* D = dst + dstOff[blk];
+    const int m = mm[blk], n = nn[blk];          // source is m x n, destination n x m
+    for (int r0 = 0; r0 < m; r0 += 64)
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+                const int r = e & 63, c = e >> 6;
+                if (r0 + r < m && c0 + c < n) tile[c][r] = S[(r0 + r) + (int64_t)(c0 + c) * m];
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+                const int c = e & 63, r = e >> 6;        // destination row c, column r
+                if (r0 + r < m && c0 + c < n) D[(c0 + c) + (int64_t)(r0 + r) * n] = tile[c][r];
+            }
+            __syncthreads();
+        }
+}
+template <typename T> void launch_transpose_dense(const T* src, T* dst, const int64_t* srcOff,
+                                                  const int64_t* dstOff, const int* m, const int* n,
+                                                  int count, hipStream_t s)
+{
+    if (count <= 0) return;
+    transpose_dense_kernel<T><<<count, 256, 0, s>>>(src, dst, srcOff, dstOff, m, n);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void swap_lowrank_kernel(const T* src, T* dst, const int64_t* srcU, const int64_t* srcV,
+                                    const int64_t* dstU, const int64_t* dstV, const int* mrows,
+                                    const int* nrows, const int* cap)
+{
+    const int blk = blockIdx.x;
+    // destination U (mrows x cap) <- source V; destination V (nrows x cap) <- source U
+    const int64_t nu = (int64_t)mrows[blk] * cap[blk], nv = (int64_t)nrows[blk] * cap[blk];
+    for (int64_t e = threadIdx.x; e < nu; e += blockDim.x) dst[dstU[blk] + e] = src[srcV[blk] + e];
+    for (int64_t e = threadIdx.x; e < nv; e += blockDim.x) dst[dstV[blk] + e] = src[srcU[blk] + e];
+}
+template <typename T> void launch_swap_lowrank(const T* src, T* dst, const int64_t* srcU,
+                                               const int64_t* srcV, const int64_t* dstU,
+                                               const int64_t* dstV, const int* mrows,
+                                               const int* nrows, const int* cap, int count,
+                                               hipStream_t s)
+{
+    if (count <= 0) return;
+    swap_lowrank_kernel<T><<<count, 256, 0, s>>>(src, dst, srcU, srcV, dstU, dstV, mrows, nrows, cap);
+    HIP_LAUNCH_CHECK();
+}
+
+__global__ void permute_int_kernel(const int* src, int* dst, const int* perm, int count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = src[perm[i]];
+}
+void launch_permute_int(const int* src, int* dst, const int* perm, int count, hipStream_t s)
+{
+    if (count <= 0) return;
+    permute_int_kernel<<<(count + 255) / 256, 256, 0, s>>>(src, dst, perm, count);
+    HIP_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Power-iteration helpers.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) sumsq_kernel(const T* x, int64_t n, double* out)
+{
+    __shared__ double red[4];
+    double s = 0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)x[e];
+        s += v * v;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
+}
+template <typename T> void launch_sumsq(const T* x, int64_t n, double* out, hipStream_t s)
+{
+    sumsq_kernel<T><<<std::min(grid_for(n), 1024), 256, 0, s>>>(x, n, out);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void normalize_kernel(T* x, int64_t n, const double* sumsq, double* logAcc, double root)
+{
+    const double ss = *sumsq;
+    const double inv = ss > 0 ? 1.0 / sqrt(ss) : 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        x[e] = (T)((double)x[e] * inv);
+    if (logAcc && blockIdx.x == 0 && threadIdx.x == 0) *logAcc += root * 0.5 * log(ss);
+}
+template <typename T> void launch_normalize(T* x, int64_t n, const double* sumsq, double* logAcc,
+                                            double root, hipStream_t s)
+{
+    normalize_kernel<T><<<grid_for(n), 256, 0, s>>>(x, n, sumsq, logAcc, root);
+    HIP_LAUNCH_CHECK();
+}
+
+__global__ void zero_double_kernel(double* x, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+void launch_zero_double(double* x, int n, hipStream_t s)
+{
+    zero_double_kernel<<<(n + 63) / 64, 64, 0, s>>>(x, n);
+    HIP_LAUNCH_CHECK();
+}
+
+// Explicit instantiations.
+#define INST(T)                                                                                       \
+    template void launch_tile_phase<T>(const Phase&, const Bases&, T, T, hipStream_t);              \
+    template void launch_qr_phase<T>(const Phase&, const Bases&, hipStream_t);                      \
+    template void launch_svd_phase<T>(const Phase&, const Bases&, TruncPolicy, hipStream_t);        \
+    template void launch_rng_phase<T>(const Phase&, const Bases&, uint64_t, hipStream_t);           \
+    template void launch_scale<T>(T*, int64_t, T, hipStream_t);                                     \
+    template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                      \
+    template void launch_gaussian<T>(T*, int64_t, uint64_t, uint64_t, hipStream_t);                 \
+    template void launch_add_diag<T>(T*, const int64_t*, const int*, int, T, hipStream_t);          \
+    template void launch_transpose_dense<T>(const T*, T*, const int64_t*, const int64_t*, const int*, \
+                                            const int*, int, hipStream_t);                          \
+    template void launch_swap_lowrank<T>(const T*, T*, const int64_t*, const int64_t*, const int64_t*, \
+                                         const int64_t*, const int*, const int*, const int*, int,   \
+                                         hipStream_t);                                              \
+    template void launch_sumsq<T>(const T*, int64_t, double*, hipStream_t);                         \
+    template void launch_normalize<T>(T*, int64_t, const double*, double*, double, hipStream_t);
+INST(double)
+INST(float)
+template void launch_convert<double, float>(const double*, float*, int64_t, hipStream_t);
+template void launch_convert<float, double>(const float*, double*, int64_t, hipStream_t);
+
+} // namespace dmhm

@@ -1,0 +1,58 @@
+// Host-side launchers for the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "plan.hpp"
+
+namespace dmhm {
+
+struct Bases {
+    void* val[NUM_SLOTS];
+    int* rank[NUM_SLOTS];
+};
+
+// Truncation policy of the SVD phase.
+struct TruncPolicy {
+    int tolMode;        // 0 fixed rank, 1 relative tolerance
+    double relTol;
+};
+
+template <typename T>
+void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s);
+template <typename T>
+void launch_qr_phase(const Phase& ph, const Bases& b, hipStream_t s);
+template <typename T>
+void launch_svd_phase(const Phase& ph, const Bases& b, TruncPolicy pol, hipStream_t s);
+template <typename T>
+void launch_rng_phase(const Phase& ph, const Bases& b, uint64_t seed, hipStream_t s);
+
+// Elementwise helpers over raw device arrays.
+template <typename T> void launch_scale(T* x, int64_t n, T alpha, hipStream_t s);
+template <typename T> void launch_fill(T* x, int64_t n, T v, hipStream_t s);
+void launch_fill_int(int* x, int64_t n, int v, hipStream_t s);
+template <typename Ts, typename Td> void launch_convert(const Ts* x, Td* y, int64_t n, hipStream_t s);
+template <typename T> void launch_gaussian(T* x, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s);
+
+// Per-leaf operations driven by small index tables on the device.
+// add alpha to the diagonal of square dense leaves: offsets + sizes.
+template <typename T> void launch_add_diag(T* vals, const int64_t* offs, const int* sizes, int count,
+                                           T alpha, hipStream_t s);
+// dst dense leaf i = transpose of src dense leaf perm[i] (m x n -> n x m).
+template <typename T> void launch_transpose_dense(const T* src, T* dst, const int64_t* srcOff,
+                                                  const int64_t* dstOff, const int* m, const int* n,
+                                                  int count, hipStream_t s);
+// dst U/V of low-rank block i = src V/U of block perm[i]; ranks copied.
+template <typename T> void launch_swap_lowrank(const T* src, T* dst, const int64_t* srcU,
+                                               const int64_t* srcV, const int64_t* dstU,
+                                               const int64_t* dstV, const int* mrows,
+                                               const int* nrows, const int* cap, int count,
+                                               hipStream_t s);
+void launch_permute_int(const int* src, int* dst, const int* perm, int count, hipStream_t s);
+
+// Vector utilities for the power iteration (EstimateTwoNorm).
+template <typename T> void launch_sumsq(const T* x, int64_t n, double* out, hipStream_t s);
+template <typename T> void launch_normalize(T* x, int64_t n, const double* sumsq, double* logAcc,
+                                            double root, hipStream_t s);
+void launch_zero_double(double* x, int n, hipStream_t s);
+
+} // namespace dmhm

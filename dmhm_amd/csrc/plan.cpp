@@ -1,0 +1,524 @@
+#include "plan.hpp"
+#include <algorithm>
+#include <stdexcept>
+
+namespace dmhm {
+
+void PhaseBuilder::finalize()
+{
+    // Stable counting sort of the pending contributions by tile id.
+    const int nt = (int)ph_.tiles.size();
+    std::vector<int> count(nt + 1, 0);
+    for (auto& p : pending_) count[p.first + 1]++;
+    for (int i = 0; i < nt; ++i) count[i + 1] += count[i];
+    const int base = (int)ph_.contribs.size();
+    ph_.contribs.resize(base + pending_.size());
+    std::vector<int> pos(count.begin(), count.end() - 1);
+    for (auto& p : pending_) ph_.contribs[base + pos[p.first]++] = p.second;
+    for (int i = 0; i < nt; ++i) {
+        ph_.tiles[i].cb = base + count[i];
+        ph_.tiles[i].ce = base + count[i + 1];
+        ph_.maxKTile = std::max(ph_.maxKTile, ph_.tiles[i].k);
+    }
+    pending_.clear();
+    pending_.shrink_to_fit();
+}
+
+static Contrib mkc(uint8_t lslot, int64_t loff, int ldl, uint8_t rslot, int64_t roff, int ldr,
+                   int q, int qrank, uint8_t qslot, uint8_t flags, float alpha)
+{
+    Contrib c;
+    c.loff = loff; c.roff = roff; c.ldl = ldl; c.ldr = ldr; c.q = q; c.qrank = qrank;
+    c.lslot = lslot; c.rslot = rslot; c.qslot = qslot; c.flags = flags; c.alpha = alpha;
+    return c;
+}
+
+Planner::PanelTiles Planner::addPanelTiles(PhaseBuilder& pb, const Operand& out, int level,
+                                           int box, int k, int krank, uint8_t kslot,
+                                           uint8_t flags)
+{
+    const Box& B = g_.boxes[level][box];
+    PanelTiles pt;
+    pt.level = level; pt.box = box;
+    pt.kChunks = (k + 63) / 64;
+    pt.tileOfAtom.resize((size_t)(B.atomEnd - B.atomBegin) * pt.kChunks);
+    for (int a = B.atomBegin; a < B.atomEnd; ++a)
+        for (int c = 0; c < pt.kChunks; ++c) {
+            Tile t{};
+            t.off = out.off + (g_.atomStart[a] - B.off) + (int64_t)c * 64 * out.ld;
+            t.ld = out.ld;
+            t.m = g_.atomSize[a];
+            t.k = std::min(64, k - 64 * c);
+            t.mrank = -1; t.moff = 0;
+            t.krank = krank; t.koff = 64 * c; t.kslot = kslot;
+            t.slot = out.slot; t.mslot = 0;
+            t.flags = flags;
+            pt.tileOfAtom[(size_t)(a - B.atomBegin) * pt.kChunks + c] = pb.addTile(t);
+        }
+    return pt;
+}
+
+void Planner::subh(PhaseBuilder& proj, PhaseBuilder& acc, uint8_t mslot, const Layout& LM, int b,
+                   bool trans, const Operand& in, int inLevel, int inBox, const PanelTiles& out,
+                   int k, int krank, uint8_t kslot, float alpha, uint8_t cflags, double* flops)
+{
+    const Box& InB = g_.boxes[inLevel][inBox];
+    const Box& OutB = g_.boxes[out.level][out.box];
+    const Block& blk = g_.blocks[b];
+    for (int li = blk.leafBegin; li < blk.leafEnd; ++li) {
+        const int lb = g_.leaves[li];
+        const Block& L = g_.blocks[lb];
+        const Box& tb = g_.tbox(lb);
+        const Box& sb = g_.sbox(lb);
+        const Box& rowB = trans ? sb : tb;
+        const Box& colB = trans ? tb : sb;
+        if (colB.off < InB.off || colB.off + colB.size > InB.off + InB.size ||
+            rowB.off < OutB.off || rowB.off + rowB.size > OutB.off + OutB.size)
+            throw std::logic_error("sub-H product operand outside its panel");
+        const int64_t inRow = colB.off - InB.off;
+        if (L.type == BT_DENSE) {
+            const int64_t D = LM.denseOff[L.denseIndex];
+            const int m = tb.size, n = sb.size;
+            for (int a = rowB.atomBegin; a < rowB.atomEnd; ++a) {
+                const int ra = g_.atomStart[a] - rowB.off;
+                for (int c = 0; c < out.kChunks; ++c) {
+                    const int tile = out.tileOfAtom[(size_t)(a - OutB.atomBegin) * out.kChunks + c];
+                    const int64_t roff = in.off + inRow + (int64_t)c * 64 * in.ld;
+                    if (!trans)
+                        acc.add(tile, mkc(mslot, D + ra, m, in.slot, roff, in.ld, n, -1, 0, cflags, alpha));
+                    else
+                        acc.add(tile, mkc(mslot, D + (int64_t)ra * m, m, in.slot, roff, in.ld, m, -1, 0,
+                                          cflags | CF_TRANS_L, alpha));
+                }
+                if (flops) *flops += 2.0 * g_.atomSize[a] * (trans ? m : n) * k;
+            }
+        } else {
+            const int li2 = L.lrIndex;
+            const int cap = LM.cap[li2];
+            const int64_t U = LM.uOff[li2], V = LM.vOff[li2];
+            const int64_t inSide = trans ? U : V;          // rows = colB
+            const int inLd = trans ? tb.size : sb.size;
+            const int64_t outSide = trans ? V : U;         // rows = rowB
+            const int outLd = trans ? sb.size : tb.size;
+            // Projection P = inSide^T * in(colB rows) : cap x k (ld cap)
+            const int64_t P = wsAlloc((int64_t)cap * k);
+            const int kCh = (k + 63) / 64, rCh = (cap + 63) / 64;
+            for (int rc = 0; rc < rCh; ++rc)
+                for (int c = 0; c < kCh; ++c) {
+                    Tile t{};
+                    t.off = P + rc * 64 + (int64_t)c * 64 * cap;
+                    t.ld = cap;
+                    t.m = std::min(64, cap - 64 * rc); t.mrank = li2; t.moff = 64 * rc; t.mslot = mslot;
+                    t.k = std::min(64, k - 64 * c); t.krank = krank; t.koff = 64 * c; t.kslot = kslot;
+                    t.slot = SLOT_WS; t.flags = 0;
+                    const int tile = proj.addTile(t);
+                    for (int a = colB.atomBegin; a < colB.atomEnd; ++a) {
+                        const int ra = g_.atomStart[a] - colB.off;
+                        proj.add(tile, mkc(mslot, inSide + ra + (int64_t)rc * 64 * inLd, inLd, in.slot,
+                                           in.off + inRow + ra + (int64_t)c * 64 * in.ld, in.ld,
+                                           g_.atomSize[a], -1, 0, CF_TRANS_L, 1.0f));
+                    }
+                }
+            if (flops) *flops += 2.0 * colB.size * cap * k;
+            for (int a = rowB.atomBegin; a < rowB.atomEnd; ++a) {
+                const int ra = g_.atomStart[a] - rowB.off;
+                for (int c = 0; c < out.kChunks; ++c) {
+                    const int tile = out.tileOfAtom[(size_t)(a - OutB.atomBegin) * out.kChunks + c];
+                    acc.add(tile, mkc(mslot, outSide + ra, outLd, SLOT_WS, P + (int64_t)c * 64 * cap, cap,
+                                      cap, li2, mslot, cflags, alpha));
+                }
+            }
+            if (flops) *flops += 2.0 * rowB.size * cap * k;
+        }
+    }
+}
+
+Plan* Planner::matvec(const Layout& L, bool trans, int k, int ldx, int ldy)
+{
+    Plan* plan = new Plan();
+    plan->phases.resize(2);
+    plan->phases[0].kind = PH_TILE;
+    plan->phases[1].kind = PH_TILE;
+    PhaseBuilder proj(plan->phases[0]), acc(plan->phases[1]);
+    Operand out{SLOT_Y, 0, ldy}, in{SLOT_X, 0, ldx};
+    PanelTiles pt = addPanelTiles(acc, out, 0, 0, k, -1, 0, TF_RUNTIME_BETA);
+    subh(proj, acc, SLOT_A, L, g_.root(), trans, in, 0, 0, pt, k, -1, 0, 1.0f, CF_RUNTIME_ALPHA,
+         &plan->flops);
+    proj.finalize();
+    acc.finalize();
+    plan->wsElems = wsUsed();
+    for (auto& ph : plan->phases) { plan->tileCount += ph.tiles.size(); plan->contribCount += ph.contribs.size(); }
+    return plan;
+}
+
+// ---------------------------------------------------------------------------
+// H*H product with randomized sum-sketch compression.
+// ---------------------------------------------------------------------------
+namespace {
+struct WFactor {            // exact low-rank term W = PU PV^T
+    Operand PU, PV;
+    int level, t, s;        // boxes of PU rows (t) and PV rows (s) at `level`
+    int rankIndex;          // rank array entry of the low-rank operand
+    uint8_t rankSlot;
+    int cap;
+};
+struct Term { int bA, bB; };
+} // namespace
+
+Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, int maxRank,
+                         int oversample)
+{
+    const Geometry& g = g_;
+    const int kS = maxRank + oversample;
+    if (kS > 64) throw std::invalid_argument("maxRank + oversample must be <= 64");
+    Plan* plan = new Plan();
+
+    // 1. Enumerate the product recursion (MultiplyHMat-incl.hpp:12-187).
+    std::vector<WFactor> W;
+    std::vector<std::vector<int>> pushdowns(g.blocks.size()), lrTerms(g.blocks.size());
+    std::vector<std::vector<Term>> nnTerms(g.blocks.size()), ddTerms(g.blocks.size());
+    struct Pending { int bA, bB; int wIndex; bool left; };
+    std::vector<Pending> wPending;
+    auto makeW = [&](int bA, int bB, int bC) -> int {
+        const Block& A = g.blocks[bA];
+        const Block& Bb = g.blocks[bB];
+        const Block& C = g.blocks[bC];
+        WFactor w;
+        w.level = C.level; w.t = C.t; w.s = C.s;
+        const Box& tb = g.boxes[C.level][C.t];
+        const Box& sb = g.boxes[C.level][C.s];
+        if (A.type == BT_LOWRANK) {
+            const int la = A.lrIndex;
+            w.cap = LA.cap[la];
+            w.PU = {SLOT_A, LA.uOff[la], tb.size};
+            w.PV = {SLOT_WS, wsAlloc((int64_t)sb.size * w.cap), sb.size};
+            w.rankIndex = la; w.rankSlot = SLOT_A;
+            wPending.push_back({bA, bB, (int)W.size(), true});
+        } else {
+            const int lb = Bb.lrIndex;
+            w.cap = LB.cap[lb];
+            w.PU = {SLOT_WS, wsAlloc((int64_t)tb.size * w.cap), tb.size};
+            w.PV = {SLOT_B, LB.vOff[lb], sb.size};
+            w.rankIndex = lb; w.rankSlot = SLOT_B;
+            wPending.push_back({bA, bB, (int)W.size(), false});
+        }
+        W.push_back(w);
+        return (int)W.size() - 1;
+    };
+    {
+        struct Tr { int a, b, c; };
+        std::vector<Tr> stack{{g.root(), g.root(), g.root()}};
+        while (!stack.empty()) {
+            Tr tr = stack.back(); stack.pop_back();
+            const Block& A = g.blocks[tr.a];
+            const Block& B = g.blocks[tr.b];
+            const Block& C = g.blocks[tr.c];
+            if (C.type == BT_NODE) {
+                if (A.type == BT_NODE && B.type == BT_NODE) {
+                    const int nc = g.nc();
+                    for (int tc = 0; tc < nc; ++tc)
+                        for (int uc = 0; uc < nc; ++uc)
+                            for (int sc = 0; sc < nc; ++sc)
+                                stack.push_back({g.child(tr.a, tc, uc), g.child(tr.b, uc, sc),
+                                                 g.child(tr.c, tc, sc)});
+                } else pushdowns[tr.c].push_back(makeW(tr.a, tr.b, tr.c));
+            } else if (C.type == BT_LOWRANK) {
+                if (A.type == BT_LOWRANK || B.type == BT_LOWRANK) lrTerms[tr.c].push_back(makeW(tr.a, tr.b, tr.c));
+                else nnTerms[tr.c].push_back({tr.a, tr.b});
+            } else {
+                if (A.type == BT_DENSE && B.type == BT_DENSE) ddTerms[tr.c].push_back({tr.a, tr.b});
+                else lrTerms[tr.c].push_back(makeW(tr.a, tr.b, tr.c));
+            }
+        }
+    }
+    // Ancestor pushdowns of every C leaf (ImportLowRank / UpdateWithLowRank
+    // distribute a low-rank W over the subtree of a NODE, HMat2d.cpp:811-959).
+    std::vector<std::vector<int>> leafW(g.blocks.size());
+    for (size_t b = 0; b < g.blocks.size(); ++b) {
+        if (pushdowns[b].empty()) continue;
+        const Block& blk = g.blocks[b];
+        for (int li = blk.leafBegin; li < blk.leafEnd; ++li) {
+            auto& dst = leafW[g.leaves[li]];
+            dst.insert(dst.end(), pushdowns[b].begin(), pushdowns[b].end());
+        }
+    }
+    for (size_t b = 0; b < g.blocks.size(); ++b) {
+        auto& dst = leafW[b];
+        dst.insert(dst.end(), lrTerms[b].begin(), lrTerms[b].end());
+    }
+
+    auto newPhase = [&](PhaseKind k) -> Phase& {
+        plan->phases.emplace_back();
+        plan->phases.back().kind = k;
+        return plan->phases.back();
+    };
+    plan->phases.reserve(16);
+
+    // Sketch panels per admissible C block.
+    struct Sk { int b; int64_t omega, Y, Z, R, G1, G2; int m, n; };
+    std::vector<Sk> sk;
+    std::unordered_map<uint64_t, int64_t> omegaOf;     // (level, s) -> panel
+    std::unordered_map<uint64_t, int64_t> t1Of;        // B block -> T1 panel
+    auto key2 = [](int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; };
+    Phase& rngPh = newPhase(PH_RNG);
+    for (int lbk : g.lrBlocks) {
+        const Block& C = g.blocks[lbk];
+        const Box& tb = g.boxes[C.level][C.t];
+        const Box& sb = g.boxes[C.level][C.s];
+        Sk s;
+        s.b = lbk; s.m = tb.size; s.n = sb.size;
+        auto it = omegaOf.find(key2(C.level, C.s));
+        if (it == omegaOf.end()) {
+            const int64_t o = wsAlloc((int64_t)sb.size * kS);
+            omegaOf[key2(C.level, C.s)] = o;
+            RngTask r{};
+            r.off = o; r.m = sb.size; r.k = kS; r.stream = key2(C.level, C.s);
+            rngPh.rng.push_back(r);
+            s.omega = o;
+        } else s.omega = it->second;
+        s.Y = wsAlloc((int64_t)tb.size * kS);
+        s.Z = wsAlloc((int64_t)sb.size * kS);
+        s.R = wsAlloc((int64_t)kS * kS);
+        s.G1 = wsAlloc((int64_t)kS * LC.cap[C.lrIndex]);
+        s.G2 = wsAlloc((int64_t)kS * LC.cap[C.lrIndex]);
+        sk.push_back(s);
+    }
+
+    // 2. W factors and T1 = B(u,s) Omega_s panels.
+    {
+        Phase& p1 = newPhase(PH_TILE);
+        Phase& a1 = newPhase(PH_TILE);
+        PhaseBuilder proj(p1), acc(a1);
+        for (const auto& wp : wPending) {
+            const WFactor& w = W[wp.wIndex];
+            const Block& A = g.blocks[wp.bA];
+            const Block& B = g.blocks[wp.bB];
+            if (wp.left) {
+                // PV (|s| x cap) = B(u,s)^T V_A, V_A rows = u box
+                PanelTiles pt = addPanelTiles(acc, w.PV, w.level, w.s, w.cap, w.rankIndex, w.rankSlot, 0);
+                Operand in{SLOT_A, LA.vOff[A.lrIndex], g.boxes[A.level][A.s].size};
+                subh(proj, acc, SLOT_B, LB, wp.bB, true, in, A.level, A.s, pt, w.cap, w.rankIndex,
+                     w.rankSlot, 1.0f, 0, &plan->flops);
+            } else {
+                // PU (|t| x cap) = A(t,u) U_B, U_B rows = u box
+                PanelTiles pt = addPanelTiles(acc, w.PU, w.level, w.t, w.cap, w.rankIndex, w.rankSlot, 0);
+                Operand in{SLOT_B, LB.uOff[B.lrIndex], g.boxes[B.level][B.t].size};
+                subh(proj, acc, SLOT_A, LA, wp.bA, false, in, B.level, B.t, pt, w.cap, w.rankIndex,
+                     w.rankSlot, 1.0f, 0, &plan->flops);
+            }
+        }
+        for (const auto& s : sk)
+            for (const Term& tm : nnTerms[s.b]) {
+                if (t1Of.count((uint64_t)tm.bB)) continue;
+                const Block& B = g.blocks[tm.bB];
+                const Box& ub = g.boxes[B.level][B.t];
+                const int64_t T1 = wsAlloc((int64_t)ub.size * kS);
+                t1Of[(uint64_t)tm.bB] = T1;
+                PanelTiles pt = addPanelTiles(acc, {SLOT_WS, T1, ub.size}, B.level, B.t, kS, -1, 0, 0);
+                Operand in{SLOT_WS, omegaOf[key2(B.level, B.s)], g.boxes[B.level][B.s].size};
+                subh(proj, acc, SLOT_B, LB, tm.bB, false, in, B.level, B.s, pt, kS, -1, 0, 1.0f, 0,
+                     &plan->flops);
+            }
+        proj.finalize();
+        acc.finalize();
+    }
+
+    // Low-rank term restricted to C block (t,s): PU rows of t, PV rows of s.
+    auto slicePU = [&](const WFactor& w, const Box& tb) {
+        return Operand{w.PU.slot, w.PU.off + (tb.off - g.boxes[w.level][w.t].off), w.PU.ld};
+    };
+    auto slicePV = [&](const WFactor& w, const Box& sbx) {
+        return Operand{w.PV.slot, w.PV.off + (sbx.off - g.boxes[w.level][w.s].off), w.PV.ld};
+    };
+    // Projection tile P = F^T X over the atoms of box `bx` (F rows = bx rows).
+    auto projTile = [&](PhaseBuilder& pb, const Operand& F, const Box& bx, const Operand& X,
+                        const WFactor& w, int64_t P) {
+        Tile t{};
+        t.off = P; t.ld = w.cap;
+        t.m = std::min(64, w.cap); t.mrank = w.rankIndex; t.moff = 0; t.mslot = w.rankSlot;
+        t.k = kS; t.krank = -1; t.slot = SLOT_WS; t.flags = 0;
+        if (w.cap > 64) throw std::invalid_argument("low-rank capacity above 64 in a product");
+        const int tile = pb.addTile(t);
+        for (int a = bx.atomBegin; a < bx.atomEnd; ++a) {
+            const int ra = g.atomStart[a] - bx.off;
+            pb.add(tile, mkc(F.slot, F.off + ra, F.ld, X.slot, X.off + ra, X.ld, g.atomSize[a], -1, 0,
+                             CF_TRANS_L, 1.0f));
+        }
+        plan->flops += 2.0 * bx.size * w.cap * kS;
+    };
+
+    // 3. Y = T Omega for admissible C blocks; dense C leaves.
+    std::vector<std::vector<int64_t>> yProj(sk.size());
+    {
+        Phase& p2 = newPhase(PH_TILE);
+        Phase& a2 = newPhase(PH_TILE);
+        PhaseBuilder proj(p2), acc(a2);
+        for (size_t i = 0; i < sk.size(); ++i) {
+            const Sk& s = sk[i];
+            const Block& C = g.blocks[s.b];
+            const Box& tb = g.boxes[C.level][C.t];
+            const Box& sb = g.boxes[C.level][C.s];
+            PanelTiles pt = addPanelTiles(acc, {SLOT_WS, s.Y, s.m}, C.level, C.t, kS, -1, 0, 0);
+            const Operand om{SLOT_WS, s.omega, s.n};
+            for (int wi : leafW[s.b]) {
+                const WFactor& w = W[wi];
+                const int64_t P = wsAlloc((int64_t)w.cap * kS);
+                projTile(proj, slicePV(w, sb), sb, om, w, P);
+                const Operand pu = slicePU(w, tb);
+                for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
+                    const int ra = g.atomStart[a] - tb.off;
+                    acc.add(pt.tileOfAtom[a - tb.atomBegin],
+                            mkc(pu.slot, pu.off + ra, pu.ld, SLOT_WS, P, w.cap, w.cap, w.rankIndex,
+                                w.rankSlot, 0, 1.0f));
+                }
+                plan->flops += 2.0 * tb.size * w.cap * kS;
+            }
+            for (const Term& tm : nnTerms[s.b]) {
+                const Block& B = g.blocks[tm.bB];
+                const Operand in{SLOT_WS, t1Of[(uint64_t)tm.bB], g.boxes[B.level][B.t].size};
+                subh(proj, acc, SLOT_A, LA, tm.bA, false, in, B.level, B.t, pt, kS, -1, 0, 1.0f, 0,
+                     &plan->flops);
+            }
+        }
+        // Dense C leaves: C = alpha (sum D_A D_B + sum PU PV^T).
+        for (int bd : g.denseBlocks) {
+            const Block& C = g.blocks[bd];
+            const Box& tb = g.boxes[C.level][C.t];
+            const Box& sb = g.boxes[C.level][C.s];
+            const Operand out{SLOT_C, LC.denseOff[C.denseIndex], tb.size};
+            PanelTiles pt = addPanelTiles(acc, out, C.level, C.t, sb.size, -1, 0, 0);
+            for (const Term& tm : ddTerms[bd]) {
+                const Block& A = g.blocks[tm.bA];
+                const Block& B = g.blocks[tm.bB];
+                const int usz = g.boxes[A.level][A.s].size;
+                for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
+                    const int ra = g.atomStart[a] - tb.off;
+                    for (int c = 0; c < pt.kChunks; ++c)
+                        acc.add(pt.tileOfAtom[(size_t)(a - tb.atomBegin) * pt.kChunks + c],
+                                mkc(SLOT_A, LA.denseOff[A.denseIndex] + ra, tb.size, SLOT_B,
+                                    LB.denseOff[B.denseIndex] + (int64_t)c * 64 * usz, usz, usz, -1, 0,
+                                    CF_RUNTIME_ALPHA, 1.0f));
+                }
+                plan->flops += 2.0 * tb.size * usz * sb.size;
+            }
+            for (int wi : leafW[bd]) {
+                const WFactor& w = W[wi];
+                const Operand pu = slicePU(w, tb), pv = slicePV(w, sb);
+                for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
+                    const int ra = g.atomStart[a] - tb.off;
+                    for (int c = 0; c < pt.kChunks; ++c)
+                        acc.add(pt.tileOfAtom[(size_t)(a - tb.atomBegin) * pt.kChunks + c],
+                                mkc(pu.slot, pu.off + ra, pu.ld, pv.slot, pv.off + 64 * c, pv.ld, w.cap,
+                                    w.rankIndex, w.rankSlot, CF_TRANS_R | CF_RUNTIME_ALPHA, 1.0f));
+                }
+                plan->flops += 2.0 * tb.size * w.cap * sb.size;
+            }
+        }
+        proj.finalize();
+        acc.finalize();
+    }
+
+    // 4. Q = qr(Y)
+    {
+        Phase& q = newPhase(PH_QR);
+        q.qrK = kS;
+        for (const auto& s : sk) {
+            QrTask t{};
+            t.off = s.Y; t.roff = -1; t.m = s.m; t.k = kS; t.slot = SLOT_WS;
+            q.qr.push_back(t);
+            plan->flops += 4.0 * s.m * kS * kS;
+        }
+    }
+
+    // 5. Z = T^T Q: low-rank terms directly, NN terms via T2 = A(t,u)^T Q.
+    {
+        Phase& p3 = newPhase(PH_TILE);
+        Phase& a3 = newPhase(PH_TILE);
+        Phase& p4 = newPhase(PH_TILE);
+        Phase& a4 = newPhase(PH_TILE);
+        PhaseBuilder proj3(p3), acc3(a3), proj4(p4), acc4(a4);
+        for (size_t i = 0; i < sk.size(); ++i) {
+            const Sk& s = sk[i];
+            const Block& C = g.blocks[s.b];
+            const Box& tb = g.boxes[C.level][C.t];
+            const Box& sb = g.boxes[C.level][C.s];
+            PanelTiles zt = addPanelTiles(acc4, {SLOT_WS, s.Z, s.n}, C.level, C.s, kS, -1, 0, 0);
+            const Operand Q{SLOT_WS, s.Y, s.m};
+            for (int wi : leafW[s.b]) {
+                const WFactor& w = W[wi];
+                const int64_t P = wsAlloc((int64_t)w.cap * kS);
+                projTile(proj3, slicePU(w, tb), tb, Q, w, P);
+                const Operand pv = slicePV(w, sb);
+                for (int a = sb.atomBegin; a < sb.atomEnd; ++a) {
+                    const int ra = g.atomStart[a] - sb.off;
+                    acc4.add(zt.tileOfAtom[a - sb.atomBegin],
+                             mkc(pv.slot, pv.off + ra, pv.ld, SLOT_WS, P, w.cap, w.cap, w.rankIndex,
+                                 w.rankSlot, 0, 1.0f));
+                }
+                plan->flops += 2.0 * sb.size * w.cap * kS;
+            }
+            for (const Term& tm : nnTerms[s.b]) {
+                const Block& A = g.blocks[tm.bA];
+                const Box& ub = g.boxes[A.level][A.s];
+                const int64_t T2 = wsAlloc((int64_t)ub.size * kS);
+                PanelTiles pt = addPanelTiles(acc3, {SLOT_WS, T2, ub.size}, A.level, A.s, kS, -1, 0, 0);
+                subh(proj3, acc3, SLOT_A, LA, tm.bA, true, Q, C.level, C.t, pt, kS, -1, 0, 1.0f, 0,
+                     &plan->flops);
+                subh(proj4, acc4, SLOT_B, LB, tm.bB, true, {SLOT_WS, T2, ub.size}, A.level, A.s, zt, kS,
+                     -1, 0, 1.0f, 0, &plan->flops);
+            }
+        }
+        proj3.finalize(); acc3.finalize(); proj4.finalize(); acc4.finalize();
+    }
+
+    // 6. QR(Z) keeping R, 7. SVD(R) with rank decision, 8. form C.U, C.V.
+    {
+        Phase& q = newPhase(PH_QR);
+        q.qrK = kS;
+        Phase& sv = newPhase(PH_SVD);
+        for (const auto& s : sk) {
+            QrTask t{};
+            t.off = s.Z; t.roff = s.R; t.m = s.n; t.k = kS; t.slot = SLOT_WS;
+            q.qr.push_back(t);
+            plan->flops += 4.0 * s.n * kS * kS;
+            const Block& C = g.blocks[s.b];
+            SvdTask v{};
+            v.roff = s.R; v.g1off = s.G1; v.g2off = s.G2; v.k = kS;
+            v.cap = LC.cap[C.lrIndex];
+            v.rankIndex = C.lrIndex;
+            v.maxKeep = std::min(std::min(maxRank, v.cap), std::min(std::min(s.m, s.n), kS));
+            sv.svd.push_back(v);
+            plan->flops += 30.0 * kS * kS * kS;
+        }
+        Phase& f = newPhase(PH_TILE);
+        PhaseBuilder acc(f);
+        for (const auto& s : sk) {
+            const Block& C = g.blocks[s.b];
+            const int cap = LC.cap[C.lrIndex];
+            const Box& tb = g.boxes[C.level][C.t];
+            const Box& sb = g.boxes[C.level][C.s];
+            PanelTiles ut = addPanelTiles(acc, {SLOT_C, LC.uOff[C.lrIndex], s.m}, C.level, C.t, cap, -1, 0, 0);
+            PanelTiles vt = addPanelTiles(acc, {SLOT_C, LC.vOff[C.lrIndex], s.n}, C.level, C.s, cap, -1, 0, 0);
+            for (int a = tb.atomBegin; a < tb.atomEnd; ++a)
+                for (int c = 0; c < ut.kChunks; ++c)
+                    acc.add(ut.tileOfAtom[(size_t)(a - tb.atomBegin) * ut.kChunks + c],
+                            mkc(SLOT_WS, s.Y + (g.atomStart[a] - tb.off), s.m, SLOT_WS,
+                                s.G1 + (int64_t)c * 64 * kS, kS, kS, -1, 0, CF_RUNTIME_ALPHA, 1.0f));
+            for (int a = sb.atomBegin; a < sb.atomEnd; ++a)
+                for (int c = 0; c < vt.kChunks; ++c)
+                    acc.add(vt.tileOfAtom[(size_t)(a - sb.atomBegin) * vt.kChunks + c],
+                            mkc(SLOT_WS, s.Z + (g.atomStart[a] - sb.off), s.n, SLOT_WS,
+                                s.G2 + (int64_t)c * 64 * kS, kS, kS, -1, 0, 0, 1.0f));
+            plan->flops += 2.0 * (s.m + s.n) * kS * cap;
+        }
+        acc.finalize();
+    }
+    plan->wsElems = wsUsed();
+    for (auto& ph : plan->phases) {
+        plan->tileCount += ph.tiles.size();
+        plan->contribCount += ph.contribs.size();
+    }
+    return plan;
+}
+
+} // namespace dmhm

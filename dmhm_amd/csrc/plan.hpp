@@ -1,0 +1,162 @@
+// Execution plans: every operation on H-matrices (H*X, H^T*X, H*H with
+// randomized compression) is lowered on the host, once per geometry and
+// storage layout, into a short sequence of device phases:
+//
+//  * TILE phases: grouped, output-owned small GEMMs. An output tile is a
+//    row atom (<= 64 rows) x <= 64 columns of some panel; it owns a list of
+//    contributions op(L) * op(R) that the kernel sums in registers and writes
+//    once (no atomics, deterministic order).
+//  * QR phases: batched Householder QR of tall-skinny panels (Q formed).
+//  * SVD phases: batched one-sided Jacobi SVD of small k x k factors with the
+//    rank decision written to the output matrix's rank array.
+//  * RNG phases: Gaussian sketch panels.
+//
+// Operands are (slot, element offset, leading dimension): the slot selects a
+// base pointer bound at launch (A, B, C matrices, workspace, user X / Y), so
+// one plan serves every matrix with the same layout and both precisions.
+// Inner dimensions and tile shapes may be "rank-indirect": read at run time
+// from a matrix's device rank array, so plans never need host round trips
+// and whole iterations can be captured in a HIP graph.
+#pragma once
+#include <cstdint>
+#include <vector>
+#include "geometry.hpp"
+
+namespace dmhm {
+
+enum Slot : uint8_t { SLOT_A = 0, SLOT_B = 1, SLOT_C = 2, SLOT_WS = 3, SLOT_X = 4, SLOT_Y = 5,
+                      NUM_SLOTS = 8 };
+
+enum ContribFlags : uint8_t { CF_TRANS_L = 1, CF_TRANS_R = 2, CF_RUNTIME_ALPHA = 4 };
+enum TileFlags : uint8_t { TF_ACCUMULATE = 1, TF_RUNTIME_BETA = 2 };
+
+// Plain-old-data records shared with the kernels.
+struct Contrib {
+    int64_t loff, roff;     // element offsets of L(0,0) and R(0,0) in their slots
+    int32_t ldl, ldr;
+    int32_t q;              // inner dimension (if qrank < 0)
+    int32_t qrank;          // rank-array index giving the inner dimension, or -1
+    uint8_t lslot, rslot, qslot, flags;
+    float alpha;
+};
+
+struct Tile {
+    int64_t off;            // element offset of the tile's (0,0) in its slot
+    int32_t ld;
+    int32_t m, k;           // static sizes (<= 64); clamps when rank-indirect
+    int32_t mrank, krank;   // rank-array index (or -1): m = min(m, rank - moff)
+    int32_t moff, koff;
+    int32_t cb, ce;         // contribution range
+    uint8_t slot, mslot, kslot, flags;
+    int32_t pad;
+};
+
+struct QrTask {              // Householder QR of an m x k panel (ld = m), Q in place
+    int64_t off;            // panel offset (slot)
+    int64_t roff;           // R output (k x k, ld k) offset in the workspace, or -1
+    int32_t m, k;
+    uint8_t slot, pad0, pad1, pad2;
+    int32_t pad;
+};
+
+struct SvdTask {             // SVD of R (k x k) -> G1 = V S (k x cap), G2 = U (k x cap)
+    int64_t roff, g1off, g2off;   // workspace offsets
+    int32_t k, cap;
+    int32_t rankIndex;      // rank slot entry written with the kept rank
+    int32_t maxKeep;        // min(maxRank, m, n, k)
+};
+
+struct RngTask {
+    int64_t off;            // workspace offset of an m x k panel (ld m)
+    int32_t m, k;
+    uint64_t stream;        // stream id mixed into the counter
+};
+
+enum PhaseKind { PH_TILE, PH_QR, PH_SVD, PH_RNG, PH_SCALE_C };
+
+struct Phase {
+    PhaseKind kind;
+    std::vector<Tile> tiles;
+    std::vector<Contrib> contribs;
+    std::vector<QrTask> qr;
+    std::vector<SvdTask> svd;
+    std::vector<RngTask> rng;
+    int qrK = 0;            // panel width of the QR phase
+    // device copies (filled by upload)
+    void* dTiles = nullptr; void* dContribs = nullptr; void* dTasks = nullptr;
+    int maxKTile = 0;       // widest tile (selects the kernel variant)
+};
+
+struct Plan {
+    std::vector<Phase> phases;
+    int64_t wsElems = 0;    // workspace elements (of the run-time precision)
+    double flops = 0;       // algorithmic flops (GEMM shapes)
+    int64_t contribCount = 0, tileCount = 0;
+    void upload();          // copy tile/contrib/task arrays to the device
+    void release();
+    ~Plan() { release(); }
+};
+
+// Builds the phases. Contributions are collected per tile and sorted at
+// finalize so that each tile owns a contiguous contribution range.
+class PhaseBuilder {
+public:
+    explicit PhaseBuilder(Phase& ph) : ph_(ph) {}
+    int addTile(const Tile& t) { ph_.tiles.push_back(t); return (int)ph_.tiles.size() - 1; }
+    void add(int tile, const Contrib& c) { pending_.push_back({tile, c}); }
+    void finalize();
+private:
+    Phase& ph_;
+    std::vector<std::pair<int, Contrib>> pending_;
+};
+
+struct Operand {            // a panel or matrix block region
+    uint8_t slot;
+    int64_t off;
+    int32_t ld;
+};
+
+// Plan builders.
+class Planner {
+public:
+    Planner(const Geometry& g) : g_(g) {}
+
+    // Y (n x k, ld ldy) := alpha op(H) X + beta Y, H in SLOT_A with layout L.
+    Plan* matvec(const Layout& L, bool trans, int k, int ldx, int ldy);
+
+    // C := alpha A B with randomized sum-sketch compression of every admissible
+    // block (sketch width kS = maxRank + oversample), A/B/C layouts given.
+    // tolMode 0: keep min(maxRank, kS) singular values; 1: keep those above
+    // relTol * sigma_0 (at most maxRank).
+    Plan* hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, int maxRank,
+                    int oversample);
+
+    int64_t wsAlloc(int64_t elems) {
+        const int64_t o = ws_;
+        ws_ += (elems + 63) & ~int64_t(63);
+        return o;
+    }
+    int64_t wsUsed() const { return ws_; }
+
+    // Sub-H product: out(|outBox| x k) (+)= op(M(b)) in(|inBox| x k). Leaves of
+    // M inside block b contribute to the tiles of `out` (registered per atom in
+    // outTiles); low-rank leaves first project into workspace in `proj`.
+    // k may be rank-indirect (krank >= 0, kslot) with static maximum k.
+    struct PanelTiles {
+        int level, box;         // the box whose atoms the panel's rows cover
+        std::vector<int> tileOfAtom; // tile id per atom of the box (atom - box.atomBegin)
+        int kChunks;            // number of 64-column chunks
+    };
+    PanelTiles addPanelTiles(PhaseBuilder& pb, const Operand& out, int level, int box, int k,
+                             int krank, uint8_t kslot, uint8_t flags);
+    void subh(PhaseBuilder& proj, PhaseBuilder& acc, uint8_t mslot, const Layout& LM, int b,
+              bool trans, const Operand& in, int inLevel, int inBox, const PanelTiles& out,
+              int k, int krank, uint8_t kslot, float alpha, uint8_t cflags, double* flops);
+    // out tiles += U(rows of t-atoms) * P for a low-rank factor pair given as
+    // operands (generic helper used for pushdowns and exact low-rank terms).
+private:
+    const Geometry& g_;
+    int64_t ws_ = 0;
+};
+
+} // namespace dmhm

@@ -1,0 +1,241 @@
+"""Python mirror of the reference's sequential H-matrix interface.
+
+``HMat2d`` / ``HMat3d`` keep the method names, argument order and meaning of
+``dmhm::HMat2d<Scalar>`` / ``HMat3d<Scalar>`` (include/dmhm/hmat2d.hpp:22-275,
+include/dmhm/hmat3d.hpp) and ``hmat_tools::EstimateTwoNorm``
+(include/dmhm/hmat_tools.hpp:1504-1544). Every call goes through the C-ABI of
+libdmhm_amd.so to the gfx950 kernels; nothing is computed in Python.
+
+Dense operands may be numpy arrays (host memory; copied through the library's
+staging buffers) or torch tensors on the GPU (passed by device pointer).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, DMHM_F64, DMHM_F32, DMHM_HOST, DMHM_DEVICE
+
+
+def build_natural_to_hierarchical_map(sizes, num_levels):
+    """HMat2d/HMat3d::BuildNaturalToHierarchicalMap (HMat2d.cpp:69-87)."""
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    out = np.zeros(int(np.prod(sizes)), dtype=np.int32)
+    check(_lib.load().dmhm_build_natural_to_hierarchical_map(
+        len(sizes), sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), num_levels,
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return out
+
+
+def _np_dtype(prec):
+    return np.float64 if prec == DMHM_F64 else np.float32
+
+
+class _HMatBase:
+    dim = None
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle) if not isinstance(handle, ctypes.c_void_p) else handle
+
+    # -- construction ----------------------------------------------------
+    @classmethod
+    def from_csr(cls, sizes, num_levels, max_rank, strongly_admissible, row_offsets, column_indices,
+                 nonzeros, precision=DMHM_F64):
+        """HMat2d(const Sparse<Scalar>&, numLevels, maxRank, strong, xSize, ySize)."""
+        _lib.init()
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        if len(sizes) != cls.dim:
+            raise ValueError("expected %d grid sizes" % cls.dim)
+        rp = np.ascontiguousarray(row_offsets, dtype=np.int32)
+        ci = np.ascontiguousarray(column_indices, dtype=np.int32)
+        nz = np.ascontiguousarray(nonzeros, dtype=np.float64)
+        ip = ctypes.POINTER(ctypes.c_int)
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hmat_create_from_csr(
+            cls.dim, sizes.ctypes.data_as(ip), num_levels, max_rank, 1 if strongly_admissible else 0,
+            precision, rp.ctypes.data_as(ip), ci.ctypes.data_as(ip),
+            nz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(out)))
+        return cls(out)
+
+    @classmethod
+    def zeros(cls, sizes, num_levels, max_rank, strongly_admissible, precision=DMHM_F64):
+        _lib.init()
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hmat_create_zero(cls.dim, sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                num_levels, max_rank, 1 if strongly_admissible else 0,
+                                                precision, ctypes.byref(out)))
+        return cls(out)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.dmhm_hmat_destroy(h)
+            self._h = None
+
+    # -- AbstractHMat fulfilments (abstract_hmat.hpp:20-93) -----------------
+    def info(self):
+        inf = _lib.Info()
+        check(_lib.load().dmhm_hmat_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def Height(self):
+        return self.info().n
+
+    Width = Height
+
+    def NumLevels(self):
+        return self.info().numLevels
+
+    def MaxRank(self):
+        return self.info().maxRank
+
+    def StronglyAdmissible(self):
+        return bool(self.info().strong)
+
+    @property
+    def precision(self):
+        return self.info().precision
+
+    def block_counts(self, max_levels=12):
+        d = np.zeros(max_levels, dtype=np.int64)
+        f = np.zeros(max_levels, dtype=np.int64)
+        lp = ctypes.POINTER(ctypes.c_longlong)
+        check(_lib.load().dmhm_hmat_block_counts(self._h, d.ctypes.data_as(lp), f.ctypes.data_as(lp),
+                                                 max_levels))
+        return d, f
+
+    def ranks(self):
+        n = self.info().numLowRank
+        out = np.zeros(n, dtype=np.int32)
+        check(_lib.load().dmhm_hmat_ranks(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(n)))
+        return out
+
+    def matvec_bytes(self, k):
+        b = ctypes.c_double()
+        f = ctypes.c_double()
+        check(_lib.load().dmhm_hmat_matvec_bytes(self._h, k, ctypes.byref(b), ctypes.byref(f)))
+        return b.value, f.value
+
+    # -- products ------------------------------------------------------------
+    def _mult(self, op, alpha, X, beta, Y):
+        prec = self.precision
+        n = self.Height()
+        if hasattr(X, "data_ptr"):          # torch tensors on the GPU
+            import torch
+            want = torch.float64 if prec == DMHM_F64 else torch.float32
+            if X.dtype != want or (Y is not None and Y.dtype != want):
+                raise TypeError("operand dtype must match the H-matrix precision")
+            Xc = X if X.dim() == 2 else X.reshape(-1, 1)
+            k = Xc.shape[1]
+            if Y is None:
+                Y = torch.zeros((k, n), dtype=want, device=X.device).t()
+                beta = 0.0
+            Yc = Y if Y.dim() == 2 else Y.reshape(-1, 1)
+            # column-major views: stride(0) == 1
+            if Xc.stride(0) != 1 or Yc.stride(0) != 1:
+                raise ValueError("torch operands must be column-major (use .t().contiguous().t())")
+            ldx = Xc.stride(1) if k > 1 else n
+            ldy = Yc.stride(1) if k > 1 else n
+            check(_lib.load().dmhm_hmat_multiply_dense(self._h, op.encode(), alpha, ctypes.c_void_p(Xc.data_ptr()),
+                                                       ldx, k, beta, ctypes.c_void_p(Yc.data_ptr()), ldy,
+                                                       DMHM_DEVICE))
+            return Y
+        dt = _np_dtype(prec)
+        vec = np.ndim(X) == 1
+        Xc = np.asfortranarray(np.reshape(X, (n, -1)), dtype=dt)
+        k = Xc.shape[1]
+        if Y is None:
+            Yc = np.zeros((n, k), dtype=dt, order="F")
+            beta = 0.0
+        else:
+            Yc = np.asfortranarray(np.reshape(Y, (n, -1)), dtype=dt)
+        check(_lib.load().dmhm_hmat_multiply_dense(self._h, op.encode(), alpha,
+                                                   Xc.ctypes.data_as(ctypes.c_void_p), n, k, beta,
+                                                   Yc.ctypes.data_as(ctypes.c_void_p), n, DMHM_HOST))
+        return Yc.reshape(-1) if vec else Yc
+
+    def Multiply(self, alpha, B, beta=None, C=None):
+        """Multiply(alpha, x|X, [beta, y|Y]) -> alpha H X (+ beta Y), or
+        Multiply(alpha, HMat B) -> C := alpha H B (MultiplyHMat-incl.hpp:12-187)."""
+        if isinstance(B, _HMatBase):
+            return self.MultiplyHMat(alpha, B)
+        return self._mult("N", alpha, B, 0.0 if beta is None else beta, C)
+
+    def TransposeMultiply(self, alpha, B, beta=None, C=None):
+        return self._mult("T", alpha, B, 0.0 if beta is None else beta, C)
+
+    AdjointMultiply = TransposeMultiply
+
+    def MultiplyHMat(self, alpha, B, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1):
+        out = ctypes.c_void_p()
+        o = _lib.HHOpts(oversample, tol_mode, rel_tol, seed)
+        check(_lib.load().dmhm_hmat_multiply_hmat(alpha, self._h, B._h, ctypes.byref(out), ctypes.byref(o)))
+        return type(self)(out)
+
+    # -- whole-matrix routines (HMat2d.cpp:331-744) --------------------------
+    def Copy(self):
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hmat_copy(self._h, ctypes.byref(out)))
+        return type(self)(out)
+
+    def Transpose(self):
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hmat_transpose(self._h, ctypes.byref(out)))
+        return type(self)(out)
+
+    Adjoint = Transpose
+
+    def Convert(self, precision):
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hmat_convert(self._h, precision, ctypes.byref(out)))
+        return type(self)(out)
+
+    def Scale(self, alpha):
+        check(_lib.load().dmhm_hmat_scale(self._h, alpha))
+
+    def SetToIdentity(self):
+        check(_lib.load().dmhm_hmat_set_to_identity(self._h))
+
+    def AddConstantToDiagonal(self, alpha):
+        check(_lib.load().dmhm_hmat_add_constant_to_diagonal(self._h, alpha))
+
+    def SetToRandom(self, seed=1):
+        check(_lib.load().dmhm_hmat_set_to_random(self._h, seed))
+
+    def EstimateTwoNorm(self, theta=1.5, confidence=6.0, seed=1):
+        est = ctypes.c_double()
+        check(_lib.load().dmhm_hmat_estimate_two_norm(self._h, theta, confidence, seed, ctypes.byref(est)))
+        return est.value
+
+    def SchulzInvert(self, numIterations=-1, theta=1.5, confidence=6.0, stopTol=1e-4, maxIterations=30,
+                     mixed=False, switchTol=1e-3, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1,
+                     raise_on_noconv=False):
+        """A := inv(A) by Schulz iteration (Invert-incl.hpp:128-196). Returns the report."""
+        o = _lib.SchulzOpts(numIterations, maxIterations, theta, confidence, stopTol, 1 if mixed else 0,
+                            switchTol, _lib.HHOpts(oversample, tol_mode, rel_tol, seed))
+        rep = _lib.SchulzReport()
+        code = _lib.load().dmhm_hmat_schulz_invert(self._h, ctypes.byref(o), ctypes.byref(rep))
+        if code == 4 and not raise_on_noconv:
+            return rep
+        check(code)
+        return rep
+
+    def to_dense(self):
+        n = self.Height()
+        out = np.zeros((n, n), order="F")
+        check(_lib.load().dmhm_hmat_to_dense(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
+        return out
+
+
+class HMat2d(_HMatBase):
+    dim = 2
+
+
+class HMat3d(_HMatBase):
+    dim = 3
+
+
+def EstimateTwoNorm(A, theta=1.5, confidence=6.0, seed=1):
+    """hmat_tools::EstimateTwoNorm (include/dmhm/hmat_tools.hpp:1504-1544)."""
+    return A.EstimateTwoNorm(theta, confidence, seed)

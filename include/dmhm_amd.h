@@ -1,0 +1,178 @@
+/*
+ * C-ABI of the MI355X-native H-matrix library (libdmhm_amd.so).
+ *
+ * Plain pointers and sizes only. Each entry point replaces a member or free
+ * function of the reference's sequential classes; the reference interface it
+ * stands for is cited beside it (paths relative to the reference tree). The
+ * header-only C++ facade include/dmhm_amd/hmat.hpp re-exposes these with the
+ * reference's class and method names (dmhm::HMat2d<Scalar>, HMat3d<Scalar>,
+ * hmat_tools::...).
+ *
+ * Conventions
+ *  - Dense operands are column-major with a leading dimension, as
+ *    Dense<Scalar> (include/dmhm/core/dense.hpp:24-88).
+ *  - Unknowns are in hierarchical ordering (dmhm_build_natural_to_hierarchical_map).
+ *  - Low-rank blocks are U V^T (transpose convention, core/low_rank.hpp:23-35).
+ *  - Every function returns DMHM_OK or an error code; dmhm_last_error()
+ *    gives the text. The reference throws only in debug builds
+ *    (std::logic_error / std::runtime_error); the facade maps codes back to
+ *    those exception types.
+ *  - Real scalars only (float and double storage, selected per handle).
+ *  - Pointer arguments flagged DMHM_DEVICE are device (HBM) pointers; with
+ *    DMHM_HOST the library copies through pinned staging buffers.
+ *  - One handle is used by one host thread at a time; calls are ordered on
+ *    the library stream and return after the GPU work has finished.
+ */
+#ifndef DMHM_AMD_H
+#define DMHM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum dmhm_status {
+    DMHM_OK = 0,
+    DMHM_EINVAL = 1,      /* bad argument (reference: std::logic_error)          */
+    DMHM_ENONCONF = 2,    /* nonconformal operands (MultiplyHMat-incl.hpp:19-25) */
+    DMHM_ESINGULAR = 3,   /* singular dense leaf                                 */
+    DMHM_ENOCONV = 4,     /* Schulz did not reach the stop tolerance             */
+    DMHM_EHIP = 5,        /* HIP runtime error                                   */
+    DMHM_ENOMEM = 6,      /* device allocation failed                            */
+    DMHM_ENODEV = 7       /* no GPU / library not initialised                    */
+};
+
+enum dmhm_precision { DMHM_F64 = 8, DMHM_F32 = 4 };
+enum dmhm_memory { DMHM_HOST = 0, DMHM_DEVICE = 1 };
+
+typedef struct dmhm_hmat_s* dmhm_hmat;
+
+/* Library set-up: binds device `device`, creates the library stream.
+ * Reference: dmhm::Initialize (src/core/global.cpp:55-94; MPI and the global
+ * RNG seeding are replaced by per-handle seeds). */
+int dmhm_init(int device);
+int dmhm_finalize(void);
+const char* dmhm_last_error(void);
+/* Device synchronisation point used by benchmarks. */
+int dmhm_synchronize(void);
+
+/* HMat2d::BuildNaturalToHierarchicalMap (src/HMat2d/HMat2d.cpp:69-87),
+ * HMat3d::BuildNaturalToHierarchicalMap (src/HMat3d/HMat3d.cpp:93-111).
+ * dim = 2 or 3, sizes[dim]; map[natural] = hierarchical. Host only. */
+int dmhm_build_natural_to_hierarchical_map(int dim, const int* sizes, int numLevels, int* map);
+
+/* HMat2d(const Sparse<Scalar>&, numLevels, maxRank, strong, xSize, ySize)
+ * (src/HMat2d/HMat2d.cpp:144-162 + ImportSparse :961-1033) and the HMat3d
+ * counterpart. CSR (host arrays) with global hierarchical indices.
+ * Admissible blocks are stored exactly (rank = number of nonzeros in the
+ * block, ConvertSubmatrix.cpp:57-119). */
+int dmhm_hmat_create_from_csr(int dim, const int* sizes, int numLevels, int maxRank, int strong,
+                              int precision, const int* rowOffsets, const int* columnIndices,
+                              const double* nonzeros, dmhm_hmat* out);
+
+/* HMat2d(numLevels, maxRank, symmetric=false, strong, xSize, ySize)
+ * (src/HMat2d/HMat2d.cpp:104-117): a zero matrix with the full block tree
+ * (low-rank blocks at rank 0, dense leaves zero). */
+int dmhm_hmat_create_zero(int dim, const int* sizes, int numLevels, int maxRank, int strong,
+                          int precision, dmhm_hmat* out);
+
+int dmhm_hmat_destroy(dmhm_hmat h);
+
+typedef struct {
+    int dim, n, numLevels, maxRank, strong, precision;
+    long long numDense, numLowRank;
+    long long storageBytes;      /* bytes of dense + U + V pools         */
+    long long sumRank;           /* sum of current low-rank block ranks  */
+    int maxBlockRank;
+} dmhm_info;
+/* Height/Width/NumLevels/MaxRank/StronglyAdmissible (hmat2d.hpp:96-104). */
+int dmhm_hmat_info(dmhm_hmat h, dmhm_info* info);
+/* Dense/low-rank counts per level (levels <= maxLevels). */
+int dmhm_hmat_block_counts(dmhm_hmat h, long long* dense, long long* lowRank, int maxLevels);
+
+/* HMat2d::SetToRandom (src/HMat2d/SetToRandom-incl.hpp:12-53): dense leaves
+ * and U, V of every admissible block at rank = maxRank filled with N(0,1),
+ * from a counter-based generator keyed by `seed`. */
+int dmhm_hmat_set_to_random(dmhm_hmat h, uint64_t seed);
+
+/* Y := alpha op(H) X + beta Y, op = 'N' or 'T' ('C' == 'T' for real data).
+ * HMat2d::Multiply / TransposeMultiply / AdjointMultiply for Dense and
+ * Vector (src/HMat2d/MultiplyDense-incl.hpp:12-262,
+ * MultiplyVector-incl.hpp:12-245). X: n x k (ldx), Y: n x k (ldy), in the
+ * handle's precision, host or device memory as `memory` says. */
+int dmhm_hmat_multiply_dense(dmhm_hmat h, char op, double alpha, const void* X, int ldx, int k,
+                             double beta, void* Y, int ldy, int memory);
+
+typedef struct {
+    int oversample;      /* sketch width = maxRank + oversample (Oversample(), global.cpp:30) */
+    int tolMode;         /* 0: fixed rank maxRank (reference); 1: keep s_i > relTol*s_0      */
+    double relTol;       /* relative tolerance for tolMode 1 (Truncation-incl.hpp:37-63)     */
+    uint64_t seed;       /* sketch seed                                                      */
+} dmhm_hh_opts;
+
+/* C := alpha A B, HMat2d::Multiply(alpha, B, C) (src/HMat2d/MultiplyHMat-incl.hpp:12-187).
+ * Every admissible block of C is the rank-truncated randomized range
+ * approximation of the exact restricted product (Gaussian sketch, Householder
+ * QR, Jacobi SVD); dense leaves are exact. *C is created (same geometry). */
+int dmhm_hmat_multiply_hmat(double alpha, dmhm_hmat A, dmhm_hmat B, dmhm_hmat* C,
+                            const dmhm_hh_opts* opts);
+
+/* HMat2d::CopyFrom / TransposeFrom / AdjointFrom / Scale / SetToIdentity /
+ * AddConstantToDiagonal (src/HMat2d/HMat2d.cpp:331-744). */
+int dmhm_hmat_copy(dmhm_hmat src, dmhm_hmat* dst);
+int dmhm_hmat_transpose(dmhm_hmat src, dmhm_hmat* dst);
+int dmhm_hmat_scale(dmhm_hmat h, double alpha);
+int dmhm_hmat_set_to_identity(dmhm_hmat h);
+int dmhm_hmat_add_constant_to_diagonal(dmhm_hmat h, double alpha);
+/* Precision conversion (FP64 <-> FP32 copies of the pools). */
+int dmhm_hmat_convert(dmhm_hmat src, int precision, dmhm_hmat* dst);
+
+/* hmat_tools::EstimateTwoNorm (include/dmhm/hmat_tools.hpp:1504-1544):
+ * k = ceil(log(0.8 sqrt(n) 10^confidence) / log(theta)) power steps on H from
+ * a Gaussian start; estimate = theta * prod ||H x_i||^(1/k). */
+int dmhm_hmat_estimate_two_norm(dmhm_hmat h, double theta, double confidence, uint64_t seed,
+                                double* estimate);
+
+typedef struct {
+    int numIterations;   /* > 0: fixed count; <= 0: stop on the residual estimate */
+    int maxIterations;   /* cap in stop mode (reference: 30, Invert-incl.hpp:145-149) */
+    double theta;        /* 1.5 */
+    double confidence;   /* 6   */
+    double stopTol;      /* stop when the estimate of ||I - X A||_2 < stopTol */
+    int mixed;           /* 0: all FP64; 1: FP32 iterations until the estimate
+                            drops below switchTol, then FP64 for the rest */
+    double switchTol;
+    dmhm_hh_opts hh;
+} dmhm_schulz_opts;
+
+typedef struct {
+    int iterations;
+    int iterationsLowPrecision;
+    double residualEstimate;     /* last estimate of ||I - X A||_2 */
+    double normEstimate;         /* estimate of ||A||_2 used for X_0 */
+    double seconds;              /* wall time of the whole inversion */
+    double history[128];         /* residual estimate per iteration */
+} dmhm_schulz_report;
+
+/* A := inv(A) by Schulz iteration X_{k+1} = (2I - X_k A) X_k from
+ * X_0 = alpha A^T, alpha = 2/est^2 (HMat2d::SchulzInvert,
+ * src/HMat2d/Invert-incl.hpp:128-196). Replaces h's contents. */
+int dmhm_hmat_schulz_invert(dmhm_hmat h, const dmhm_schulz_opts* opts, dmhm_schulz_report* report);
+
+/* Expand to a dense n x n column-major host matrix (HMat2d::Print multiplies
+ * by the identity, HMat2d.cpp:233-248). For tests on small matrices. */
+int dmhm_hmat_to_dense(dmhm_hmat h, double* out, int ld);
+
+/* Read the per-block ranks (one int per admissible block, tree DFS order). */
+int dmhm_hmat_ranks(dmhm_hmat h, int* ranks, int count);
+
+/* Bytes streamed by one H*X with k right-hand sides (algorithmic: every
+ * dense and low-rank factor entry at its current rank, plus X and Y). */
+int dmhm_hmat_matvec_bytes(dmhm_hmat h, int k, double* bytes, double* flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMHM_AMD_H */
