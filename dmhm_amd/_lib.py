@@ -51,7 +51,7 @@ class SchulzOpts(ctypes.Structure):
     _fields_ = [("numIterations", ctypes.c_int), ("maxIterations", ctypes.c_int),
                 ("theta", ctypes.c_double), ("confidence", ctypes.c_double),
                 ("stopTol", ctypes.c_double), ("mixed", ctypes.c_int), ("switchTol", ctypes.c_double),
-                ("hh", HHOpts)]
+                ("hh", HHOpts), ("stallStop", ctypes.c_int)]
 
 
 class SchulzReport(ctypes.Structure):
@@ -73,6 +73,11 @@ SIGNATURES = {
     "dmhm_last_error": (ctypes.c_char_p, []),
     "dmhm_synchronize": (ctypes.c_int, []),
     "dmhm_build_natural_to_hierarchical_map": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _ip]),
+    "dmhm_geometry_block_counts": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _llp, _llp,
+                                                  ctypes.c_int]),
+    "dmhm_geometry_leaves": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int,
+                                            _ip]),
+    "dmhm_geometry_product_terms": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _llp]),
     "dmhm_hmat_create_from_csr": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_int, _ip, _ip, _dp, _hp]),
     "dmhm_hmat_create_zero": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -96,6 +101,8 @@ SIGNATURES = {
     "dmhm_hmat_to_dense": (ctypes.c_int, [_h, _dp, ctypes.c_int]),
     "dmhm_hmat_ranks": (ctypes.c_int, [_h, _ip, ctypes.c_int]),
     "dmhm_hmat_matvec_bytes": (ctypes.c_int, [_h, ctypes.c_int, _dp, _dp]),
+    "dmhm_profile_matvec": (ctypes.c_int, [_h, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "dmhm_profile_hh": (ctypes.c_int, [ctypes.c_double, _h, _h, ctypes.POINTER(HHOpts), _dp, _dp]),
 }
 
 _lib = None

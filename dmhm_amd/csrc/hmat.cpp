@@ -440,6 +440,11 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
     scale_hmat(X.get(), alpha);
     int it = 0, itLow = 0;
     double e = -1;
+    std::vector<double> hist;
+    auto stalled = [&]() {
+        const size_t n = hist.size();
+        return o.stallStop && n >= 3 && hist[n - 1] < 0.1 && hist[n - 1] > 0.5 * hist[n - 3];
+    };
     for (int kk = 0; kk < numIt; ++kk) {
         it = kk + 1;
         if (low) ++itLow;
@@ -450,18 +455,20 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
         if (stop) {
             e = estimate_two_norm(Z.get(), o.theta, o.confidence, seed++);
             if (rep && kk < 128) rep->history[kk] = e;
-            if (std::fabs(e) < o.stopTol && !low) done = true;
+            hist.push_back(e);
+            if (!low && (std::fabs(e) < o.stopTol || stalled())) done = true;
         }
         add_diag(Z.get(), 1.0);
         hh.seed = seed++;
         std::unique_ptr<dmhm_hmat_s> Xn(multiply_hmat(1.0, Z.get(), X.get(), hh));
         X = std::move(Xn);
         if (done) break;
-        if (low && stop && std::fabs(e) < o.switchTol) {
+        if (low && stop && (std::fabs(e) < o.switchTol || stalled())) {
             // Switch to FP64 for the remaining iterations.
             X.reset(copy_hmat(X.get(), DMHM_F64));
             Acur = A;
             low = false;
+            hist.clear();
         }
     }
     if (low) { X.reset(copy_hmat(X.get(), DMHM_F64)); }
@@ -478,7 +485,8 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
         rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     if (stop && !(std::fabs(e) < o.stopTol))
-        throw Error(DMHM_ENOCONV, "Schulz iteration did not reach the stop tolerance");
+        throw Error(DMHM_ENOCONV, "Schulz iteration did not reach the stop tolerance (estimate " +
+                                  std::to_string(e) + ")");
 }
 
 template <typename T>
@@ -614,6 +622,122 @@ static void matvec_bytes(dmhm_hmat_s* h, int k, double* bytes, double* flops)
     if (flops) *flops = 2.0 * k * entries;
 }
 
+template <typename T>
+static void run_phase(Phase& ph, const Bases& b, T alpha, T beta, TruncPolicy pol, uint64_t seed)
+{
+    hipStream_t s = g_lib.stream;
+    switch (ph.kind) {
+    case PH_TILE: launch_tile_phase<T>(ph, b, alpha, beta, s); break;
+    case PH_QR: launch_qr_phase<T>(ph, b, s); break;
+    case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
+    case PH_RNG: launch_rng_phase<T>(ph, b, seed, s); break;
+    default: break;
+    }
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() { HIPCHECK(hipEventCreate(&a)); HIPCHECK(hipEventCreate(&b)); }
+    ~EventPair() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    float ms() { float t = 0; HIPCHECK(hipEventElapsedTime(&t, a, b)); return t; }
+};
+
+static void profile_matvec(dmhm_hmat_s* h, int k, int reps, double* ms, double* bytes)
+{
+    const Geometry& g = h->geo();
+    const Layout& L = *h->lay;
+    const int n = g.n;
+    GeoCtx& gc = *h->gc;
+    const std::string key = "mv:" + std::to_string(L.id) + ":0:" + std::to_string(k) + ":" + std::to_string(n) +
+                            ":" + std::to_string(n);
+    Plan& p = get_plan(gc, key, [&] { Planner pl(gc.geo); return pl.matvec(L, false, k, n, n); });
+    ensure_ws(p, h->esize());
+    g_lib.stageX.ensure((size_t)n * k * h->esize());
+    g_lib.stageY.ensure((size_t)n * k * h->esize());
+    if (h->prec == DMHM_F64) launch_gaussian<double>((double*)g_lib.stageX.ptr, (int64_t)n * k, 5, 5, g_lib.stream);
+    else launch_gaussian<float>((float*)g_lib.stageX.ptr, (int64_t)n * k, 5, 5, g_lib.stream);
+    Bases b = bases_for(h, nullptr, nullptr, g_lib.stageX.ptr, g_lib.stageY.ptr);
+    EventPair e0, e1;
+    double tot = 0, t0 = 0, t1 = 0;
+    for (int r = -2; r < reps; ++r) {
+        HIPCHECK(hipEventRecord(e0.a, g_lib.stream));
+        if (h->prec == DMHM_F64) run_phase<double>(p.phases[0], b, 1.0, 0.0, {0, 0}, 0);
+        else run_phase<float>(p.phases[0], b, 1.0f, 0.0f, {0, 0}, 0);
+        HIPCHECK(hipEventRecord(e0.b, g_lib.stream));
+        HIPCHECK(hipEventRecord(e1.a, g_lib.stream));
+        if (h->prec == DMHM_F64) run_phase<double>(p.phases[1], b, 1.0, 0.0, {0, 0}, 0);
+        else run_phase<float>(p.phases[1], b, 1.0f, 0.0f, {0, 0}, 0);
+        HIPCHECK(hipEventRecord(e1.b, g_lib.stream));
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        if (r >= 0) {
+            const double a = e0.ms(), c = e1.ms();
+            float whole = 0;
+            HIPCHECK(hipEventElapsedTime(&whole, e0.a, e1.b));
+            t0 += a; t1 += c; tot += whole;
+        }
+    }
+    ms[0] = tot / reps; ms[1] = t0 / reps; ms[2] = t1 / reps;
+    // Algorithmic bytes at the current ranks.
+    std::vector<int> r(L.nLowRank());
+    if (!r.empty()) HIPCHECK(hipMemcpy(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
+    double vEnt = 0, uEnt = 0, dEnt = 0;
+    for (int bd : g.denseBlocks) dEnt += (double)g.tbox(bd).size * g.sbox(bd).size;
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        uEnt += (double)g.tbox(g.lrBlocks[i]).size * r[i];
+        vEnt += (double)g.sbox(g.lrBlocks[i]).size * r[i];
+    }
+    const double es = (double)h->esize();
+    bytes[0] = es * (vEnt + (double)n * k);
+    bytes[1] = es * (dEnt + uEnt + (double)n * k);
+}
+
+static void profile_hh(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, const dmhm_hh_opts& o, double* ms,
+                       double* flops)
+{
+    if (A->gc != B->gc) throw Error(DMHM_ENONCONF, "Attempted nonconformal matrix-matrix multiply");
+    if (A->prec != B->prec) throw Error(DMHM_EINVAL, "operands must share a precision");
+    GeoCtx& gc = *A->gc;
+    const int maxRank = A->maxRank;
+    auto LC = std_layout(gc, maxRank);
+    std::unique_ptr<dmhm_hmat_s> C(alloc_hmat(A->gc, LC, A->prec, maxRank));
+    const int over = o.oversample > 0 ? o.oversample : 4;
+    const std::string key = "hh:" + std::to_string(A->lay->id) + ":" + std::to_string(B->lay->id) + ":" +
+                            std::to_string(LC->id) + ":" + std::to_string(maxRank) + ":" + std::to_string(over);
+    Plan& p = get_plan(gc, key, [&] { Planner pl(gc.geo); return pl.hhProduct(*A->lay, *B->lay, *LC, maxRank, over); });
+    ensure_ws(p, A->esize());
+    Bases b = bases_for(A, B, C.get(), nullptr, nullptr);
+    TruncPolicy pol{o.tolMode, o.relTol};
+    std::vector<EventPair> ev(p.phases.size());
+    EventPair all;
+    HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    HIPCHECK(hipEventRecord(all.a, g_lib.stream));
+    for (size_t i = 0; i < p.phases.size(); ++i) {
+        HIPCHECK(hipEventRecord(ev[i].a, g_lib.stream));
+        if (A->prec == DMHM_F64) run_phase<double>(p.phases[i], b, alpha, 0.0, pol, o.seed);
+        else run_phase<float>(p.phases[i], b, (float)alpha, 0.0f, pol, o.seed);
+        HIPCHECK(hipEventRecord(ev[i].b, g_lib.stream));
+    }
+    HIPCHECK(hipEventRecord(all.b, g_lib.stream));
+    HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    for (int i = 0; i < 6; ++i) ms[i] = 0;
+    flops[0] = flops[1] = 0;
+    ms[0] = all.ms();
+    for (size_t i = 0; i < p.phases.size(); ++i) {
+        const double t = ev[i].ms();
+        switch (p.phases[i].kind) {
+        case PH_TILE:
+            ms[1] += t;
+            flops[0] += p.phases[i].flops;
+            if (t > ms[5]) { ms[5] = t; flops[1] = p.phases[i].flops; }
+            break;
+        case PH_QR: ms[2] += t; break;
+        case PH_SVD: ms[3] += t; break;
+        case PH_RNG: ms[4] += t; break;
+        default: break;
+        }
+    }
+}
+
 static int guard(const std::function<void()>& fn)
 {
     try {
@@ -683,6 +807,51 @@ int dmhm_build_natural_to_hierarchical_map(int dim, const int* sizes, int numLev
         Geometry g(dim, sizes, numLevels, true);
         auto m = g.naturalToHierarchical();
         std::copy(m.begin(), m.end(), map);
+    });
+}
+
+int dmhm_geometry_block_counts(int dim, const int* sizes, int numLevels, int strong, long long* dense,
+                               long long* lowRank, int maxLevels)
+{
+    return guard([&] {
+        if (!sizes || !dense || !lowRank) throw Error(DMHM_EINVAL, "null pointer");
+        Geometry g(dim, sizes, numLevels, strong != 0);
+        std::vector<long> d, f;
+        g.counts(d, f);
+        for (int l = 0; l < maxLevels; ++l) {
+            dense[l] = l < (int)d.size() ? d[l] : 0;
+            lowRank[l] = l < (int)f.size() ? f[l] : 0;
+        }
+    });
+}
+
+int dmhm_geometry_leaves(int dim, const int* sizes, int numLevels, int strong, int* out, int maxLeaves,
+                         int* count)
+{
+    return guard([&] {
+        if (!sizes || !count) throw Error(DMHM_EINVAL, "null pointer");
+        Geometry g(dim, sizes, numLevels, strong != 0);
+        *count = (int)g.leaves.size();
+        for (int i = 0; i < (int)g.leaves.size() && i < maxLeaves && out; ++i) {
+            const Block& b = g.blocks[g.leaves[i]];
+            const Box& tb = g.tbox(g.leaves[i]);
+            const Box& sb = g.sbox(g.leaves[i]);
+            int* r = out + 7 * (size_t)i;
+            r[0] = b.type == BT_DENSE ? 1 : 0; r[1] = b.level;
+            r[2] = tb.off; r[3] = tb.size; r[4] = sb.off; r[5] = sb.size;
+            r[6] = b.type == BT_DENSE ? b.denseIndex : b.lrIndex;
+        }
+    });
+}
+
+int dmhm_geometry_product_terms(int dim, const int* sizes, int numLevels, int strong, long long* out)
+{
+    return guard([&] {
+        if (!sizes || !out) throw Error(DMHM_EINVAL, "null pointer");
+        Geometry g(dim, sizes, numLevels, strong != 0);
+        ProductTerms t = enumerateProductTerms(g);
+        out[0] = t.sketchTerms; out[1] = t.ddToLowRank; out[2] = t.ddToDense;
+        out[3] = t.lowRankTerms; out[4] = t.pushdownTerms;
     });
 }
 
@@ -909,6 +1078,27 @@ int dmhm_hmat_ranks(dmhm_hmat h, int* ranks, int count)
         const int n = std::min(count, h->lay->nLowRank());
         HIPCHECK(hipStreamSynchronize(g_lib.stream));
         if (n > 0) HIPCHECK(hipMemcpy(ranks, h->ranks, n * sizeof(int), hipMemcpyDeviceToHost));
+    });
+}
+
+int dmhm_profile_matvec(dmhm_hmat h, int k, int reps, double* ms, double* bytes)
+{
+    return guard([&] {
+        require_ready();
+        if (!h || !ms || !bytes) throw Error(DMHM_EINVAL, "null pointer");
+        if (k < 1 || reps < 1) throw Error(DMHM_EINVAL, "k and reps must be positive");
+        profile_matvec(h, k, reps, ms, bytes);
+    });
+}
+
+int dmhm_profile_hh(double alpha, dmhm_hmat A, dmhm_hmat B, const dmhm_hh_opts* opts, double* ms, double* flops)
+{
+    return guard([&] {
+        require_ready();
+        if (!A || !B || !ms || !flops) throw Error(DMHM_EINVAL, "null pointer");
+        dmhm_hh_opts o{4, 0, 1e-10, 1};
+        if (opts) o = *opts;
+        profile_hh(alpha, A, B, o, ms, flops);
     });
 }
 

@@ -28,7 +28,7 @@ namespace dmhm {
 // output rows i and columns j = g + 4*kk, kk < KPT. L (m x q) and R (q x k)
 // are staged through LDS in chunks of 64 along q; R is pre-scaled by alpha.
 // ---------------------------------------------------------------------------
-template <typename T, int KPT>
+template <typename T, int KPT, int ROLE>
 __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restrict__ tiles,
                                                    const Contrib* __restrict__ cs, T alpha, T beta)
 {
@@ -111,11 +111,20 @@ void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStre
     const Tile* t = static_cast<const Tile*>(ph.dTiles);
     const Contrib* c = static_cast<const Contrib*>(ph.dContribs);
     const int kmax = ph.maxKTile;
-    if (kmax <= 4) tile_kernel<T, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-    else if (kmax <= 8) tile_kernel<T, 2><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-    else if (kmax <= 16) tile_kernel<T, 4><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-    else if (kmax <= 32) tile_kernel<T, 8><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-    else tile_kernel<T, 16><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    // ROLE only distinguishes projection and accumulation launches in profiles.
+    if (ph.role == 1) {
+        if (kmax <= 4) tile_kernel<T, 1, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+        else if (kmax <= 8) tile_kernel<T, 2, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+        else if (kmax <= 16) tile_kernel<T, 4, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+        else if (kmax <= 32) tile_kernel<T, 8, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+        else tile_kernel<T, 16, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    } else {
+        if (kmax <= 4) tile_kernel<T, 1, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+        else if (kmax <= 8) tile_kernel<T, 2, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+        else if (kmax <= 16) tile_kernel<T, 4, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+        else if (kmax <= 32) tile_kernel<T, 8, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+        else tile_kernel<T, 16, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    }
     HIP_LAUNCH_CHECK();
 }
 

@@ -16,9 +16,11 @@ void PhaseBuilder::finalize()
     std::vector<int> pos(count.begin(), count.end() - 1);
     for (auto& p : pending_) ph_.contribs[base + pos[p.first]++] = p.second;
     for (int i = 0; i < nt; ++i) {
-        ph_.tiles[i].cb = base + count[i];
-        ph_.tiles[i].ce = base + count[i + 1];
-        ph_.maxKTile = std::max(ph_.maxKTile, ph_.tiles[i].k);
+        Tile& t = ph_.tiles[i];
+        t.cb = base + count[i];
+        t.ce = base + count[i + 1];
+        ph_.maxKTile = std::max(ph_.maxKTile, t.k);
+        for (int c = t.cb; c < t.ce; ++c) ph_.flops += 2.0 * t.m * ph_.contribs[c].q * t.k;
     }
     pending_.clear();
     pending_.shrink_to_fit();
@@ -139,6 +141,7 @@ Plan* Planner::matvec(const Layout& L, bool trans, int k, int ldx, int ldy)
     plan->phases.resize(2);
     plan->phases[0].kind = PH_TILE;
     plan->phases[1].kind = PH_TILE;
+    plan->phases[0].role = 1;
     PhaseBuilder proj(plan->phases[0]), acc(plan->phases[1]);
     Operand out{SLOT_Y, 0, ldy}, in{SLOT_X, 0, ldx};
     PanelTiles pt = addPanelTiles(acc, out, 0, 0, k, -1, 0, TF_RUNTIME_BETA);
@@ -154,6 +157,56 @@ Plan* Planner::matvec(const Layout& L, bool trans, int k, int ldx, int ldy)
 // ---------------------------------------------------------------------------
 // H*H product with randomized sum-sketch compression.
 // ---------------------------------------------------------------------------
+ProductTerms enumerateProductTerms(const Geometry& g)
+{
+    ProductTerms T;
+    const size_t nb = g.blocks.size();
+    T.pushdowns.resize(nb); T.lrTerms.resize(nb); T.nnTerms.resize(nb); T.ddTerms.resize(nb);
+    auto addW = [&](int a, int b, int c) {
+        T.w.push_back({a, b, c, g.blocks[a].type == BT_LOWRANK});
+        return (int)T.w.size() - 1;
+    };
+    struct Tr { int a, b, c; };
+    std::vector<Tr> stack{{g.root(), g.root(), g.root()}};
+    while (!stack.empty()) {
+        const Tr tr = stack.back();
+        stack.pop_back();
+        const Block& A = g.blocks[tr.a];
+        const Block& B = g.blocks[tr.b];
+        const Block& C = g.blocks[tr.c];
+        if (C.type == BT_NODE) {
+            if (A.type == BT_NODE && B.type == BT_NODE) {
+                const int nc = g.nc();
+                for (int tc = 0; tc < nc; ++tc)
+                    for (int uc = 0; uc < nc; ++uc)
+                        for (int sc = 0; sc < nc; ++sc)
+                            stack.push_back({g.child(tr.a, tc, uc), g.child(tr.b, uc, sc), g.child(tr.c, tc, sc)});
+            } else {
+                T.pushdowns[tr.c].push_back(addW(tr.a, tr.b, tr.c));
+                T.pushdownTerms++;
+            }
+        } else if (C.type == BT_LOWRANK) {
+            if (A.type == BT_LOWRANK || B.type == BT_LOWRANK) {
+                T.lrTerms[tr.c].push_back(addW(tr.a, tr.b, tr.c));
+                T.lowRankTerms++;
+            } else {
+                T.nnTerms[tr.c].push_back({tr.a, tr.b});
+                if (A.type == BT_DENSE) T.ddToLowRank++;
+                else T.sketchTerms++;
+            }
+        } else {
+            if (A.type == BT_DENSE && B.type == BT_DENSE) {
+                T.ddTerms[tr.c].push_back({tr.a, tr.b});
+                T.ddToDense++;
+            } else {
+                T.lrTerms[tr.c].push_back(addW(tr.a, tr.b, tr.c));
+                T.lowRankTerms++;
+            }
+        }
+    }
+    return T;
+}
+
 namespace {
 struct WFactor {            // exact low-rank term W = PU PV^T
     Operand PU, PV;
@@ -162,7 +215,6 @@ struct WFactor {            // exact low-rank term W = PU PV^T
     uint8_t rankSlot;
     int cap;
 };
-struct Term { int bA, bB; };
 } // namespace
 
 Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, int maxRank,
@@ -173,64 +225,40 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     if (kS > 64) throw std::invalid_argument("maxRank + oversample must be <= 64");
     Plan* plan = new Plan();
 
-    // 1. Enumerate the product recursion (MultiplyHMat-incl.hpp:12-187).
-    std::vector<WFactor> W;
-    std::vector<std::vector<int>> pushdowns(g.blocks.size()), lrTerms(g.blocks.size());
-    std::vector<std::vector<Term>> nnTerms(g.blocks.size()), ddTerms(g.blocks.size());
+    // 1. Enumerate the product recursion and give every low-rank term its
+    //    factor panels.
+    ProductTerms terms = enumerateProductTerms(g);
+    std::vector<WFactor> W(terms.w.size());
     struct Pending { int bA, bB; int wIndex; bool left; };
     std::vector<Pending> wPending;
-    auto makeW = [&](int bA, int bB, int bC) -> int {
-        const Block& A = g.blocks[bA];
-        const Block& Bb = g.blocks[bB];
-        const Block& C = g.blocks[bC];
-        WFactor w;
+    for (size_t wi = 0; wi < terms.w.size(); ++wi) {
+        const auto& tw = terms.w[wi];
+        const Block& A = g.blocks[tw.bA];
+        const Block& Bb = g.blocks[tw.bB];
+        const Block& C = g.blocks[tw.bC];
+        WFactor& w = W[wi];
         w.level = C.level; w.t = C.t; w.s = C.s;
         const Box& tb = g.boxes[C.level][C.t];
         const Box& sb = g.boxes[C.level][C.s];
-        if (A.type == BT_LOWRANK) {
+        if (tw.leftLowRank) {
             const int la = A.lrIndex;
             w.cap = LA.cap[la];
             w.PU = {SLOT_A, LA.uOff[la], tb.size};
             w.PV = {SLOT_WS, wsAlloc((int64_t)sb.size * w.cap), sb.size};
             w.rankIndex = la; w.rankSlot = SLOT_A;
-            wPending.push_back({bA, bB, (int)W.size(), true});
         } else {
             const int lb = Bb.lrIndex;
             w.cap = LB.cap[lb];
             w.PU = {SLOT_WS, wsAlloc((int64_t)tb.size * w.cap), tb.size};
             w.PV = {SLOT_B, LB.vOff[lb], sb.size};
             w.rankIndex = lb; w.rankSlot = SLOT_B;
-            wPending.push_back({bA, bB, (int)W.size(), false});
         }
-        W.push_back(w);
-        return (int)W.size() - 1;
-    };
-    {
-        struct Tr { int a, b, c; };
-        std::vector<Tr> stack{{g.root(), g.root(), g.root()}};
-        while (!stack.empty()) {
-            Tr tr = stack.back(); stack.pop_back();
-            const Block& A = g.blocks[tr.a];
-            const Block& B = g.blocks[tr.b];
-            const Block& C = g.blocks[tr.c];
-            if (C.type == BT_NODE) {
-                if (A.type == BT_NODE && B.type == BT_NODE) {
-                    const int nc = g.nc();
-                    for (int tc = 0; tc < nc; ++tc)
-                        for (int uc = 0; uc < nc; ++uc)
-                            for (int sc = 0; sc < nc; ++sc)
-                                stack.push_back({g.child(tr.a, tc, uc), g.child(tr.b, uc, sc),
-                                                 g.child(tr.c, tc, sc)});
-                } else pushdowns[tr.c].push_back(makeW(tr.a, tr.b, tr.c));
-            } else if (C.type == BT_LOWRANK) {
-                if (A.type == BT_LOWRANK || B.type == BT_LOWRANK) lrTerms[tr.c].push_back(makeW(tr.a, tr.b, tr.c));
-                else nnTerms[tr.c].push_back({tr.a, tr.b});
-            } else {
-                if (A.type == BT_DENSE && B.type == BT_DENSE) ddTerms[tr.c].push_back({tr.a, tr.b});
-                else lrTerms[tr.c].push_back(makeW(tr.a, tr.b, tr.c));
-            }
-        }
+        wPending.push_back({tw.bA, tw.bB, (int)wi, tw.leftLowRank});
     }
+    const auto& pushdowns = terms.pushdowns;
+    const auto& lrTerms = terms.lrTerms;
+    const auto& nnTerms = terms.nnTerms;
+    const auto& ddTerms = terms.ddTerms;
     // Ancestor pushdowns of every C leaf (ImportLowRank / UpdateWithLowRank
     // distribute a low-rank W over the subtree of a NODE, HMat2d.cpp:811-959).
     std::vector<std::vector<int>> leafW(g.blocks.size());
@@ -287,6 +315,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     // 2. W factors and T1 = B(u,s) Omega_s panels.
     {
         Phase& p1 = newPhase(PH_TILE);
+        p1.role = 1;
         Phase& a1 = newPhase(PH_TILE);
         PhaseBuilder proj(p1), acc(a1);
         for (const auto& wp : wPending) {
@@ -308,15 +337,15 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             }
         }
         for (const auto& s : sk)
-            for (const Term& tm : nnTerms[s.b]) {
-                if (t1Of.count((uint64_t)tm.bB)) continue;
-                const Block& B = g.blocks[tm.bB];
+            for (const auto& tm : nnTerms[s.b]) {
+                if (t1Of.count((uint64_t)tm.second)) continue;
+                const Block& B = g.blocks[tm.second];
                 const Box& ub = g.boxes[B.level][B.t];
                 const int64_t T1 = wsAlloc((int64_t)ub.size * kS);
-                t1Of[(uint64_t)tm.bB] = T1;
+                t1Of[(uint64_t)tm.second] = T1;
                 PanelTiles pt = addPanelTiles(acc, {SLOT_WS, T1, ub.size}, B.level, B.t, kS, -1, 0, 0);
                 Operand in{SLOT_WS, omegaOf[key2(B.level, B.s)], g.boxes[B.level][B.s].size};
-                subh(proj, acc, SLOT_B, LB, tm.bB, false, in, B.level, B.s, pt, kS, -1, 0, 1.0f, 0,
+                subh(proj, acc, SLOT_B, LB, tm.second, false, in, B.level, B.s, pt, kS, -1, 0, 1.0f, 0,
                      &plan->flops);
             }
         proj.finalize();
@@ -351,6 +380,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     std::vector<std::vector<int64_t>> yProj(sk.size());
     {
         Phase& p2 = newPhase(PH_TILE);
+        p2.role = 1;
         Phase& a2 = newPhase(PH_TILE);
         PhaseBuilder proj(p2), acc(a2);
         for (size_t i = 0; i < sk.size(); ++i) {
@@ -373,10 +403,10 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 }
                 plan->flops += 2.0 * tb.size * w.cap * kS;
             }
-            for (const Term& tm : nnTerms[s.b]) {
-                const Block& B = g.blocks[tm.bB];
-                const Operand in{SLOT_WS, t1Of[(uint64_t)tm.bB], g.boxes[B.level][B.t].size};
-                subh(proj, acc, SLOT_A, LA, tm.bA, false, in, B.level, B.t, pt, kS, -1, 0, 1.0f, 0,
+            for (const auto& tm : nnTerms[s.b]) {
+                const Block& B = g.blocks[tm.second];
+                const Operand in{SLOT_WS, t1Of[(uint64_t)tm.second], g.boxes[B.level][B.t].size};
+                subh(proj, acc, SLOT_A, LA, tm.first, false, in, B.level, B.t, pt, kS, -1, 0, 1.0f, 0,
                      &plan->flops);
             }
         }
@@ -387,9 +417,9 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             const Box& sb = g.boxes[C.level][C.s];
             const Operand out{SLOT_C, LC.denseOff[C.denseIndex], tb.size};
             PanelTiles pt = addPanelTiles(acc, out, C.level, C.t, sb.size, -1, 0, 0);
-            for (const Term& tm : ddTerms[bd]) {
-                const Block& A = g.blocks[tm.bA];
-                const Block& B = g.blocks[tm.bB];
+            for (const auto& tm : ddTerms[bd]) {
+                const Block& A = g.blocks[tm.first];
+                const Block& B = g.blocks[tm.second];
                 const int usz = g.boxes[A.level][A.s].size;
                 for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
                     const int ra = g.atomStart[a] - tb.off;
@@ -433,8 +463,10 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     // 5. Z = T^T Q: low-rank terms directly, NN terms via T2 = A(t,u)^T Q.
     {
         Phase& p3 = newPhase(PH_TILE);
+        p3.role = 1;
         Phase& a3 = newPhase(PH_TILE);
         Phase& p4 = newPhase(PH_TILE);
+        p4.role = 1;
         Phase& a4 = newPhase(PH_TILE);
         PhaseBuilder proj3(p3), acc3(a3), proj4(p4), acc4(a4);
         for (size_t i = 0; i < sk.size(); ++i) {
@@ -457,14 +489,14 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 }
                 plan->flops += 2.0 * sb.size * w.cap * kS;
             }
-            for (const Term& tm : nnTerms[s.b]) {
-                const Block& A = g.blocks[tm.bA];
+            for (const auto& tm : nnTerms[s.b]) {
+                const Block& A = g.blocks[tm.first];
                 const Box& ub = g.boxes[A.level][A.s];
                 const int64_t T2 = wsAlloc((int64_t)ub.size * kS);
                 PanelTiles pt = addPanelTiles(acc3, {SLOT_WS, T2, ub.size}, A.level, A.s, kS, -1, 0, 0);
-                subh(proj3, acc3, SLOT_A, LA, tm.bA, true, Q, C.level, C.t, pt, kS, -1, 0, 1.0f, 0,
+                subh(proj3, acc3, SLOT_A, LA, tm.first, true, Q, C.level, C.t, pt, kS, -1, 0, 1.0f, 0,
                      &plan->flops);
-                subh(proj4, acc4, SLOT_B, LB, tm.bB, true, {SLOT_WS, T2, ub.size}, A.level, A.s, zt, kS,
+                subh(proj4, acc4, SLOT_B, LB, tm.second, true, {SLOT_WS, T2, ub.size}, A.level, A.s, zt, kS,
                      -1, 0, 1.0f, 0, &plan->flops);
             }
         }

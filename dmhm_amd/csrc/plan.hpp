@@ -85,6 +85,8 @@ struct Phase {
     // device copies (filled by upload)
     void* dTiles = nullptr; void* dContribs = nullptr; void* dTasks = nullptr;
     int maxKTile = 0;       // widest tile (selects the kernel variant)
+    int role = 0;           // 0 accumulation, 1 projection (profiling identity)
+    double flops = 0;       // algorithmic flops (2 m q k per contribution at static sizes)
 };
 
 struct Plan {
@@ -115,6 +117,26 @@ struct Operand {            // a panel or matrix block region
     int64_t off;
     int32_t ld;
 };
+
+// Terms of C = A B for three H-matrices of one geometry, from the reference's
+// recursion (MultiplyHMat-incl.hpp:12-380): a triple (A(t,u), B(u,s), C(t,s))
+// recurses while all three are NODEs; otherwise it is
+//  * a low-rank term (A or B low rank): exact factors W = PU PV^T, added to an
+//    admissible or dense C block, or pushed down into a C NODE
+//    (ImportLowRank / UpdateWithLowRank, HMat2d.cpp:811-959);
+//  * a NODE*NODE term into an admissible C block (the randomized sketch case,
+//    hmat_tools.hpp:1657-1746);
+//  * a dense*dense term into an admissible C block (truncated D*D,
+//    MultiplyMatrix.cpp:405-449) or into a dense C leaf.
+struct ProductTerms {
+    struct W { int bA, bB, bC; bool leftLowRank; };
+    std::vector<W> w;                              // all low-rank terms
+    std::vector<std::vector<int>> pushdowns;       // per C NODE block: w ids
+    std::vector<std::vector<int>> lrTerms;         // per C leaf block: w ids
+    std::vector<std::vector<std::pair<int, int>>> nnTerms, ddTerms;  // per C block (bA, bB)
+    long sketchTerms = 0, ddToLowRank = 0, ddToDense = 0, pushdownTerms = 0, lowRankTerms = 0;
+};
+ProductTerms enumerateProductTerms(const Geometry& g);
 
 // Plan builders.
 class Planner {

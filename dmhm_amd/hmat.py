@@ -27,6 +27,45 @@ def build_natural_to_hierarchical_map(sizes, num_levels):
     return out
 
 
+def geometry_block_counts(sizes, num_levels, strongly_admissible, max_levels=12):
+    """Dense / low-rank block counts per level of the block tree (host only)."""
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    d = np.zeros(max_levels, dtype=np.int64)
+    f = np.zeros(max_levels, dtype=np.int64)
+    lp = ctypes.POINTER(ctypes.c_longlong)
+    check(_lib.load().dmhm_geometry_block_counts(len(sizes), sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                 num_levels, 1 if strongly_admissible else 0,
+                                                 d.ctypes.data_as(lp), f.ctypes.data_as(lp), max_levels))
+    return d, f
+
+
+def geometry_leaves(sizes, num_levels, strongly_admissible):
+    """Leaf blocks (DFS order) as an int array of rows
+    (type: 0 low rank / 1 dense, level, row_off, rows, col_off, cols, index)."""
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    cnt = ctypes.c_int()
+    check(_lib.load().dmhm_geometry_leaves(len(sizes), sizes.ctypes.data_as(ip), num_levels,
+                                           1 if strongly_admissible else 0, None, 0, ctypes.byref(cnt)))
+    out = np.zeros((cnt.value, 7), dtype=np.int32)
+    check(_lib.load().dmhm_geometry_leaves(len(sizes), sizes.ctypes.data_as(ip), num_levels,
+                                           1 if strongly_admissible else 0, out.ctypes.data_as(ip), cnt.value,
+                                           ctypes.byref(cnt)))
+    return out
+
+
+def geometry_product_terms(sizes, num_levels, strongly_admissible):
+    """Term counts of one H*H product on this geometry (host only): dict with
+    sketch (NODE*NODE into admissible), dd_to_lowrank, dd_to_dense, lowrank,
+    pushdown."""
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    out = np.zeros(5, dtype=np.int64)
+    check(_lib.load().dmhm_geometry_product_terms(len(sizes), sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                  num_levels, 1 if strongly_admissible else 0,
+                                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+    return dict(zip(["sketch", "dd_to_lowrank", "dd_to_dense", "lowrank", "pushdown"], out.tolist()))
+
+
 def _np_dtype(prec):
     return np.float64 if prec == DMHM_F64 else np.float32
 
@@ -210,16 +249,33 @@ class _HMatBase:
 
     def SchulzInvert(self, numIterations=-1, theta=1.5, confidence=6.0, stopTol=1e-4, maxIterations=30,
                      mixed=False, switchTol=1e-3, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1,
-                     raise_on_noconv=False):
+                     raise_on_noconv=False, stall_stop=False):
         """A := inv(A) by Schulz iteration (Invert-incl.hpp:128-196). Returns the report."""
         o = _lib.SchulzOpts(numIterations, maxIterations, theta, confidence, stopTol, 1 if mixed else 0,
-                            switchTol, _lib.HHOpts(oversample, tol_mode, rel_tol, seed))
+                            switchTol, _lib.HHOpts(oversample, tol_mode, rel_tol, seed), 1 if stall_stop else 0)
         rep = _lib.SchulzReport()
         code = _lib.load().dmhm_hmat_schulz_invert(self._h, ctypes.byref(o), ctypes.byref(rep))
         if code == 4 and not raise_on_noconv:
             return rep
         check(code)
         return rep
+
+    def profile_matvec(self, k=1, reps=20):
+        """HIP-event timing of H*X on the library stream: (ms, bytes) arrays."""
+        ms = np.zeros(3)
+        by = np.zeros(2)
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(_lib.load().dmhm_profile_matvec(self._h, k, reps, ms.ctypes.data_as(dp), by.ctypes.data_as(dp)))
+        return ms, by
+
+    def profile_hh(self, B, alpha=1.0, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1):
+        ms = np.zeros(6)
+        fl = np.zeros(2)
+        dp = ctypes.POINTER(ctypes.c_double)
+        o = _lib.HHOpts(oversample, tol_mode, rel_tol, seed)
+        check(_lib.load().dmhm_profile_hh(alpha, self._h, B._h, ctypes.byref(o), ms.ctypes.data_as(dp),
+                                          fl.ctypes.data_as(dp)))
+        return ms, fl
 
     def to_dense(self):
         n = self.Height()

@@ -170,6 +170,15 @@ def small_operator(n, num_levels, dim=2, smooth=True, seed=0):
                 vals=vals, map=hmap)
 
 
+def rect_operator(sizes, num_levels, seed=0):
+    """Operator on a rectangular grid (per-axis sizes) with a = 1 + 0.5 U[0,1)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = 1.0 + 0.5 * rng.random([s + 2 for s in sizes[::-1]])
+    rowptr, colind, vals, sz, hmap = flux_operator(a, num_levels)
+    return dict(dim=len(sizes), sizes=sz, num_levels=num_levels, rowptr=rowptr, colind=colind, vals=vals,
+                map=hmap)
+
+
 def csr_to_dense(rowptr, colind, vals, n):
     D = np.zeros((n, n))
     for i in range(n):

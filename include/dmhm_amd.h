@@ -63,6 +63,28 @@ int dmhm_synchronize(void);
  * dim = 2 or 3, sizes[dim]; map[natural] = hierarchical. Host only. */
 int dmhm_build_natural_to_hierarchical_map(int dim, const int* sizes, int numLevels, int* map);
 
+/* Block counts per level of the tree HMat2d/HMat3d builds for a geometry
+ * (ImportSparse recursion, src/HMat2d/HMat2d.cpp:961-1033; admissibility
+ * :790-809). Host only; no device needed. */
+int dmhm_geometry_block_counts(int dim, const int* sizes, int numLevels, int strong,
+                               long long* dense, long long* lowRank, int maxLevels);
+
+/* Leaf blocks of the tree in DFS order, 7 ints each: type (0 low rank,
+ * 1 dense), level, row offset, rows, column offset, columns, index within
+ * its type (the order of dmhm_hmat_ranks). Returns the count through *count;
+ * writes at most maxLeaves records. Host only. */
+int dmhm_geometry_leaves(int dim, const int* sizes, int numLevels, int strong, int* out,
+                         int maxLeaves, int* count);
+
+/* Term counts of one H*H product C = A B on a geometry, from the product
+ * recursion (src/HMat2d/MultiplyHMat-incl.hpp:12-380): out[0] NODE*NODE terms
+ * into admissible blocks (the randomized-sketch case), out[1] dense*dense
+ * terms into admissible blocks, out[2] dense*dense terms into dense leaves,
+ * out[3] low-rank terms into admissible or dense blocks, out[4] low-rank
+ * terms pushed into NODE blocks. Host only. */
+int dmhm_geometry_product_terms(int dim, const int* sizes, int numLevels, int strong,
+                                long long* out);
+
 /* HMat2d(const Sparse<Scalar>&, numLevels, maxRank, strong, xSize, ySize)
  * (src/HMat2d/HMat2d.cpp:144-162 + ImportSparse :961-1033) and the HMat3d
  * counterpart. CSR (host arrays) with global hierarchical indices.
@@ -145,6 +167,8 @@ typedef struct {
                             drops below switchTol, then FP64 for the rest */
     double switchTol;
     dmhm_hh_opts hh;
+    int stallStop;       /* 1: also stop (or leave low precision) once the estimate
+                            is below 0.1 and no longer halves over two iterations */
 } dmhm_schulz_opts;
 
 typedef struct {
@@ -171,6 +195,23 @@ int dmhm_hmat_ranks(dmhm_hmat h, int* ranks, int count);
 /* Bytes streamed by one H*X with k right-hand sides (algorithmic: every
  * dense and low-rank factor entry at its current rank, plus X and Y). */
 int dmhm_hmat_matvec_bytes(dmhm_hmat h, int k, double* bytes, double* flops);
+
+/* Measurement support (no counterpart in the reference; used by bench.py).
+ * Runs `reps` products H*X with k right-hand sides on library-owned device
+ * buffers and times each launch with HIP events on the library stream:
+ *   ms[0] mean time per product, ms[1] mean projection launch, ms[2] mean
+ *   accumulation launch;
+ *   bytes[0] algorithmic bytes of the projection launch (V factors at their
+ *   ranks + X), bytes[1] of the accumulation launch (dense leaves + U factors
+ *   + Y). */
+int dmhm_profile_matvec(dmhm_hmat h, int k, int reps, double* ms, double* bytes);
+
+/* Times one H*H product alpha A B (result discarded) launch by launch:
+ *   ms[0] total, ms[1] all tile launches, ms[2] QR, ms[3] SVD, ms[4] RNG,
+ *   ms[5] the longest tile launch; flops[0] algorithmic flops of all tile
+ *   launches, flops[1] of the longest tile launch. */
+int dmhm_profile_hh(double alpha, dmhm_hmat A, dmhm_hmat B, const dmhm_hh_opts* opts, double* ms,
+                    double* flops);
 
 #ifdef __cplusplus
 }
