@@ -194,11 +194,12 @@ class _HMatBase:
                                                    Yc.ctypes.data_as(ctypes.c_void_p), n, DMHM_HOST))
         return Yc.reshape(-1) if vec else Yc
 
-    def Multiply(self, alpha, B, beta=None, C=None):
+    def Multiply(self, alpha, B, beta=None, C=None, **hh_options):
         """Multiply(alpha, x|X, [beta, y|Y]) -> alpha H X (+ beta Y), or
-        Multiply(alpha, HMat B) -> C := alpha H B (MultiplyHMat-incl.hpp:12-187)."""
+        Multiply(alpha, HMat B, **hh_options) -> C := alpha H B
+        (MultiplyHMat-incl.hpp:12-187)."""
         if isinstance(B, _HMatBase):
-            return self.MultiplyHMat(alpha, B)
+            return self.MultiplyHMat(alpha, B, **hh_options)
         return self._mult("N", alpha, B, 0.0 if beta is None else beta, C)
 
     def TransposeMultiply(self, alpha, B, beta=None, C=None):
