@@ -172,6 +172,7 @@ static void run_plan(Plan& p, const Bases& b, T alpha, T beta, TruncPolicy pol, 
         case PH_QR: launch_qr_phase<T>(ph, b, s); break;
         case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
         case PH_RNG: launch_rng_phase<T>(ph, b, seed, s); break;
+        case PH_MV_ZERO: case PH_MV_PROJ: case PH_MV_ACC: launch_mv_phase<T>(ph, b, alpha, beta, s); break;
         default: break;
         }
     }
@@ -200,9 +201,14 @@ static void multiply_dense(dmhm_hmat_s* h, bool trans, double alpha, const void*
                            double beta, void* Y, int ldy)
 {
     GeoCtx& gc = *h->gc;
-    const std::string key = "mv:" + std::to_string(h->lay->id) + ":" + std::to_string(trans) + ":" +
-                            std::to_string(k) + ":" + std::to_string(ldx) + ":" + std::to_string(ldy);
-    Plan& p = get_plan(gc, key, [&] { Planner pl(gc.geo); return pl.matvec(*h->lay, trans, k, ldx, ldy); });
+    const bool stream = !trans && k <= 8;
+    const std::string key = std::string(stream ? "mvs:" : "mv:") + std::to_string(h->lay->id) + ":" +
+                            std::to_string(trans) + ":" + std::to_string(k) + ":" + std::to_string(ldx) + ":" +
+                            std::to_string(ldy);
+    Plan& p = get_plan(gc, key, [&] {
+        Planner pl(gc.geo);
+        return stream ? pl.matvecStream(*h->lay, k, ldx, ldy) : pl.matvec(*h->lay, trans, k, ldx, ldy);
+    });
     ensure_ws(p, h->esize());
     Bases b = bases_for(h, nullptr, nullptr, const_cast<void*>(X), Y);
     if (h->prec == DMHM_F64) run_plan<double>(p, b, alpha, beta, {0, 0}, 0);
@@ -631,6 +637,7 @@ static void run_phase(Phase& ph, const Bases& b, T alpha, T beta, TruncPolicy po
     case PH_QR: launch_qr_phase<T>(ph, b, s); break;
     case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
     case PH_RNG: launch_rng_phase<T>(ph, b, seed, s); break;
+    case PH_MV_ZERO: case PH_MV_PROJ: case PH_MV_ACC: launch_mv_phase<T>(ph, b, alpha, beta, s); break;
     default: break;
     }
 }
@@ -648,9 +655,14 @@ static void profile_matvec(dmhm_hmat_s* h, int k, int reps, double* ms, double* 
     const Layout& L = *h->lay;
     const int n = g.n;
     GeoCtx& gc = *h->gc;
-    const std::string key = "mv:" + std::to_string(L.id) + ":0:" + std::to_string(k) + ":" + std::to_string(n) +
-                            ":" + std::to_string(n);
-    Plan& p = get_plan(gc, key, [&] { Planner pl(gc.geo); return pl.matvec(L, false, k, n, n); });
+    const bool stream = k <= 8;
+    const std::string key = std::string(stream ? "mvs:" : "mv:") + std::to_string(L.id) + ":0:" +
+                            std::to_string(k) + ":" + std::to_string(n) + ":" + std::to_string(n);
+    Plan& p = get_plan(gc, key, [&] {
+        Planner pl(gc.geo);
+        return stream ? pl.matvecStream(L, k, n, n) : pl.matvec(L, false, k, n, n);
+    });
+    const size_t last = p.phases.size() - 1;
     ensure_ws(p, h->esize());
     g_lib.stageX.ensure((size_t)n * k * h->esize());
     g_lib.stageY.ensure((size_t)n * k * h->esize());
@@ -661,12 +673,14 @@ static void profile_matvec(dmhm_hmat_s* h, int k, int reps, double* ms, double* 
     double tot = 0, t0 = 0, t1 = 0;
     for (int r = -2; r < reps; ++r) {
         HIPCHECK(hipEventRecord(e0.a, g_lib.stream));
-        if (h->prec == DMHM_F64) run_phase<double>(p.phases[0], b, 1.0, 0.0, {0, 0}, 0);
-        else run_phase<float>(p.phases[0], b, 1.0f, 0.0f, {0, 0}, 0);
+        for (size_t i = 0; i < last; ++i) {
+            if (h->prec == DMHM_F64) run_phase<double>(p.phases[i], b, 1.0, 0.0, {0, 0}, 0);
+            else run_phase<float>(p.phases[i], b, 1.0f, 0.0f, {0, 0}, 0);
+        }
         HIPCHECK(hipEventRecord(e0.b, g_lib.stream));
         HIPCHECK(hipEventRecord(e1.a, g_lib.stream));
-        if (h->prec == DMHM_F64) run_phase<double>(p.phases[1], b, 1.0, 0.0, {0, 0}, 0);
-        else run_phase<float>(p.phases[1], b, 1.0f, 0.0f, {0, 0}, 0);
+        if (h->prec == DMHM_F64) run_phase<double>(p.phases[last], b, 1.0, 0.0, {0, 0}, 0);
+        else run_phase<float>(p.phases[last], b, 1.0f, 0.0f, {0, 0}, 0);
         HIPCHECK(hipEventRecord(e1.b, g_lib.stream));
         HIPCHECK(hipStreamSynchronize(g_lib.stream));
         if (r >= 0) {
@@ -1131,6 +1145,17 @@ void Plan::upload()
         if (ph.kind == PH_QR && !ph.qr.empty()) { src = ph.qr.data(); bytes = ph.qr.size() * sizeof(QrTask); }
         if (ph.kind == PH_SVD && !ph.svd.empty()) { src = ph.svd.data(); bytes = ph.svd.size() * sizeof(SvdTask); }
         if (ph.kind == PH_RNG && !ph.rng.empty()) { src = ph.rng.data(); bytes = ph.rng.size() * sizeof(RngTask); }
+        if (ph.kind == PH_MV_PROJ && !ph.mvProj.empty()) {
+            src = ph.mvProj.data(); bytes = ph.mvProj.size() * sizeof(MvProjItem);
+        }
+        if (ph.kind == PH_MV_ACC && !ph.mvAtoms.empty()) {
+            src = ph.mvAtoms.data(); bytes = ph.mvAtoms.size() * sizeof(MvAtom);
+            HIPCHECK(hipMalloc(&ph.dAux, std::max<size_t>(ph.mvContribs.size(), 1) * sizeof(MvContrib)));
+            if (!ph.mvContribs.empty())
+                HIPCHECK(hipMemcpy(ph.dAux, ph.mvContribs.data(), ph.mvContribs.size() * sizeof(MvContrib),
+                                   hipMemcpyHostToDevice));
+            std::vector<MvContrib>().swap(ph.mvContribs);
+        }
         if (src) {
             HIPCHECK(hipMalloc(&ph.dTasks, bytes));
             HIPCHECK(hipMemcpy(ph.dTasks, src, bytes, hipMemcpyHostToDevice));
@@ -1146,7 +1171,8 @@ void Plan::release()
         if (ph.dTiles) (void)hipFree(ph.dTiles);
         if (ph.dContribs) (void)hipFree(ph.dContribs);
         if (ph.dTasks) (void)hipFree(ph.dTasks);
-        ph.dTiles = ph.dContribs = ph.dTasks = nullptr;
+        if (ph.dAux) (void)hipFree(ph.dAux);
+        ph.dTiles = ph.dContribs = ph.dTasks = ph.dAux = nullptr;
     }
 }
 } // namespace dmhm

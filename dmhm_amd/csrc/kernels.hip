@@ -129,7 +129,7 @@ void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStre
 }
 
 // ---------------------------------------------------------------------------
-// Batched Householder QR (one workgroup per panel, FP64 arithmetic).
+// Wave helpers
 // ---------------------------------------------------------------------------
 __device__ inline double wave_sum(double v)
 {
@@ -137,6 +137,179 @@ __device__ inline double wave_sum(double v)
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+__device__ inline float wave_sum(float v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+// Broadcast lane `l` (wave-uniform) of v to the whole wave.
+__device__ inline float bcast(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline double bcast(double v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// ---------------------------------------------------------------------------
+// Streaming H*X (K right-hand sides, HBM-bound).
+// mv_proj: one wave per (low-rank block, chunk of <= 512 rows):
+//   P(j, kk) (+)= sum_r V(r, j) X(r, kk); lanes stream consecutive rows of V
+//   (coalesced), partial sums reduced across the wave at the end.
+// mv_acc: one workgroup per output row atom; its 4 waves split the atom's
+//   contributions (dense leaves D X, low-rank U P); lane i owns row i and reads
+//   L(i, p) coalesced along i; the R column is held one value per lane and
+//   broadcast with readlane; one LDS reduction across the waves at the end.
+// ---------------------------------------------------------------------------
+template <typename T, int K>
+__global__ void __launch_bounds__(256) mv_proj_kernel(Bases b, const MvProjItem* __restrict__ items, int n,
+                                                      int ldx)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int id = blockIdx.x * 4 + w;
+    if (id >= n) return;
+    const MvProjItem it = items[id];
+    const int rank = min(it.cap, b.rank[SLOT_A][it.rankIdx]);
+    if (rank <= 0) return;
+    const T* __restrict__ V = reinterpret_cast<const T*>(b.val[SLOT_A]) + it.voff;
+    const T* __restrict__ X = reinterpret_cast<const T*>(b.val[SLOT_X]) + it.xrow;
+    T* P = reinterpret_cast<T*>(b.val[SLOT_WS]) + it.poff;
+    for (int j0 = 0; j0 < rank; j0 += 16) {
+        T acc[16][K];
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) acc[jj][kk] = T(0);
+        for (int r = lane; r < it.rows; r += 64) {
+            T xv[K];
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) xv[kk] = X[r + (int64_t)kk * ldx];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                if (j0 + jj < rank) {
+                    const T v = V[r + (int64_t)(j0 + jj) * it.ldv];
+#pragma unroll
+                    for (int kk = 0; kk < K; ++kk) acc[jj][kk] += v * xv[kk];
+                }
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            if (j0 + jj >= rank) break;
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) {
+                const T sum = wave_sum(acc[jj][kk]);
+                if (lane == 0) {
+                    T* dst = P + (j0 + jj) + (int64_t)kk * it.cap;
+                    if (it.atomic) atomicAdd(dst, sum);
+                    else *dst = sum;
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int K>
+__global__ void __launch_bounds__(256) mv_acc_kernel(Bases b, const MvAtom* __restrict__ atoms,
+                                                     const MvContrib* __restrict__ cs, int ldx, int ldy,
+                                                     T alpha, T beta)
+{
+    __shared__ T red[4][K][64];
+    const MvAtom at = atoms[blockIdx.x];
+    const int w = threadIdx.x >> 6, i = threadIdx.x & 63;
+    const bool row = i < at.m;
+    T acc[K];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) acc[kk] = T(0);
+    const T* __restrict__ H = reinterpret_cast<const T*>(b.val[SLOT_A]);
+    const T* __restrict__ X = reinterpret_cast<const T*>(b.val[SLOT_X]);
+    const T* __restrict__ WS = reinterpret_cast<const T*>(b.val[SLOT_WS]);
+    for (int c = at.cb + w; c < at.ce; c += 4) {
+        const MvContrib C = cs[c];
+        int q = C.q;
+        if (C.kind == 1) q = min(q, b.rank[SLOT_A][C.rankIdx]);
+        const T* __restrict__ L = H + C.loff;
+        const T* __restrict__ R = (C.kind == 0 ? X : WS) + C.roff;
+        const int ldr = C.kind == 0 ? ldx : C.ldr;
+        for (int p0 = 0; p0 < q; p0 += 64) {
+            const int qc = min(64, q - p0);
+            T rv[K];
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) rv[kk] = (i < qc) ? R[p0 + i + (int64_t)kk * ldr] : T(0);
+            const T* __restrict__ Lp = L + i + (int64_t)p0 * C.ld;
+            int p = 0;
+            for (; p + 16 <= qc; p += 16) {
+                T l[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) l[u] = row ? Lp[(int64_t)(p + u) * C.ld] : T(0);
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+#pragma unroll
+                    for (int kk = 0; kk < K; ++kk) acc[kk] += l[u] * bcast(rv[kk], p + u);
+            }
+            for (; p < qc; ++p) {
+                const T l = row ? Lp[(int64_t)p * C.ld] : T(0);
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) acc[kk] += l * bcast(rv[kk], p);
+            }
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) red[w][kk][i] = acc[kk];
+    __syncthreads();
+    if (w == 0 && row) {
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const T sum = red[0][kk][i] + red[1][kk][i] + red[2][kk][i] + red[3][kk][i];
+            T* y = reinterpret_cast<T*>(b.val[SLOT_Y]) + at.yrow + i + (int64_t)kk * ldy;
+            *y = (beta != T(0) ? beta * *y : T(0)) + alpha * sum;
+        }
+    }
+}
+
+template <typename T, int K>
+static void launch_mv_k(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s)
+{
+    if (ph.kind == PH_MV_PROJ) {
+        const int n = (int)ph.mvProj.size();
+        if (n) mv_proj_kernel<T, K><<<(n + 3) / 4, 256, 0, s>>>(b, static_cast<const MvProjItem*>(ph.dTasks), n,
+                                                               ph.ldx);
+    } else {
+        const int n = (int)ph.mvAtoms.size();
+        if (n) mv_acc_kernel<T, K><<<n, 256, 0, s>>>(b, static_cast<const MvAtom*>(ph.dTasks),
+                                                     static_cast<const MvContrib*>(ph.dAux), ph.ldx, ph.ldy,
+                                                     alpha, beta);
+    }
+}
+
+template <typename T>
+void launch_mv_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s)
+{
+    if (ph.kind == PH_MV_ZERO) {
+        launch_fill<T>(reinterpret_cast<T*>(b.val[SLOT_WS]) + ph.zeroOff, ph.zeroCount, T(0), s);
+        return;
+    }
+    switch (ph.mvK) {
+    case 1: launch_mv_k<T, 1>(ph, b, alpha, beta, s); break;
+    case 2: launch_mv_k<T, 2>(ph, b, alpha, beta, s); break;
+    case 3: launch_mv_k<T, 3>(ph, b, alpha, beta, s); break;
+    case 4: launch_mv_k<T, 4>(ph, b, alpha, beta, s); break;
+    case 5: launch_mv_k<T, 5>(ph, b, alpha, beta, s); break;
+    case 6: launch_mv_k<T, 6>(ph, b, alpha, beta, s); break;
+    case 7: launch_mv_k<T, 7>(ph, b, alpha, beta, s); break;
+    case 8: launch_mv_k<T, 8>(ph, b, alpha, beta, s); break;
+    default: throw std::runtime_error("streaming H*X supports 1..8 columns");
+    }
+    HIP_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Batched Householder QR (one workgroup per panel, FP64 arithmetic).
+// ---------------------------------------------------------------------------
 
 // dots[c] = sum_{i > j} A(i, j) * A(i, c) for c in [c0, k)
 template <typename T>
@@ -612,6 +785,7 @@ void launch_zero_double(double* x, int n, hipStream_t s)
     template void launch_qr_phase<T>(const Phase&, const Bases&, hipStream_t);                      \
     template void launch_svd_phase<T>(const Phase&, const Bases&, TruncPolicy, hipStream_t);        \
     template void launch_rng_phase<T>(const Phase&, const Bases&, uint64_t, hipStream_t);           \
+    template void launch_mv_phase<T>(const Phase&, const Bases&, T, T, hipStream_t);                \
     template void launch_scale<T>(T*, int64_t, T, hipStream_t);                                     \
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                      \
     template void launch_gaussian<T>(T*, int64_t, uint64_t, uint64_t, hipStream_t);                 \

@@ -24,6 +24,8 @@ void launch_qr_phase(const Phase& ph, const Bases& b, hipStream_t s);
 template <typename T>
 void launch_svd_phase(const Phase& ph, const Bases& b, TruncPolicy pol, hipStream_t s);
 template <typename T>
+void launch_mv_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s);
+template <typename T>
 void launch_rng_phase(const Phase& ph, const Bases& b, uint64_t seed, hipStream_t s);
 
 // Elementwise helpers over raw device arrays.

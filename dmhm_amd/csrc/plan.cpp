@@ -154,6 +154,77 @@ Plan* Planner::matvec(const Layout& L, bool trans, int k, int ldx, int ldy)
     return plan;
 }
 
+Plan* Planner::matvecStream(const Layout& L, int k, int ldx, int ldy)
+{
+    const Geometry& g = g_;
+    Plan* plan = new Plan();
+    plan->phases.resize(3);
+    Phase& z = plan->phases[0];
+    Phase& pr = plan->phases[1];
+    Phase& ac = plan->phases[2];
+    z.kind = PH_MV_ZERO; pr.kind = PH_MV_PROJ; ac.kind = PH_MV_ACC;
+    for (Phase* ph : {&z, &pr, &ac}) { ph->mvK = k; ph->ldx = ldx; ph->ldy = ldy; }
+    // P per low-rank block: cap x k.
+    std::vector<int64_t> P(g.lrBlocks.size());
+    int64_t pTotal = 0;
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        P[i] = wsAlloc((int64_t)L.cap[i] * k);
+        pTotal = wsUsed();
+    }
+    z.zeroOff = 0; z.zeroCount = pTotal;
+    const int chunkAtoms = 8;
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        const int b = g.lrBlocks[i];
+        const Box& sb = g.sbox(b);
+        const int natoms = sb.atomEnd - sb.atomBegin;
+        const bool atomic = natoms > chunkAtoms;
+        for (int a0 = sb.atomBegin; a0 < sb.atomEnd; a0 += chunkAtoms) {
+            const int a1 = std::min(sb.atomEnd, a0 + chunkAtoms);
+            const int r0 = g.atomStart[a0] - sb.off;
+            const int rows = g.atomStart[a1 - 1] + g.atomSize[a1 - 1] - g.atomStart[a0];
+            MvProjItem it{};
+            it.voff = L.vOff[i] + r0; it.ldv = sb.size; it.rows = rows;
+            it.xrow = sb.off + r0; it.poff = P[i]; it.rankIdx = (int)i; it.cap = L.cap[i];
+            it.atomic = atomic ? 1 : 0;
+            pr.mvProj.push_back(it);
+        }
+        plan->flops += 2.0 * sb.size * L.cap[i] * k;
+    }
+    // Contributions per output atom.
+    std::vector<std::vector<MvContrib>> per(g.atomStart.size());
+    for (int b : g.leaves) {
+        const Block& B = g.blocks[b];
+        const Box& tb = g.tbox(b);
+        const Box& sb = g.sbox(b);
+        for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
+            const int ra = g.atomStart[a] - tb.off;
+            MvContrib c{};
+            if (B.type == BT_DENSE) {
+                c.kind = 0; c.loff = L.denseOff[B.denseIndex] + ra; c.ld = tb.size;
+                c.q = sb.size; c.rankIdx = -1; c.roff = sb.off; c.ldr = ldx;
+                plan->flops += 2.0 * g.atomSize[a] * sb.size * k;
+            } else {
+                c.kind = 1; c.loff = L.uOff[B.lrIndex] + ra; c.ld = tb.size;
+                c.q = L.cap[B.lrIndex]; c.rankIdx = B.lrIndex; c.roff = P[B.lrIndex]; c.ldr = L.cap[B.lrIndex];
+                plan->flops += 2.0 * g.atomSize[a] * L.cap[B.lrIndex] * k;
+            }
+            per[a].push_back(c);
+        }
+    }
+    for (size_t a = 0; a < per.size(); ++a) {
+        MvAtom at{};
+        at.yrow = g.atomStart[a]; at.m = g.atomSize[a];
+        at.cb = (int)ac.mvContribs.size();
+        ac.mvContribs.insert(ac.mvContribs.end(), per[a].begin(), per[a].end());
+        at.ce = (int)ac.mvContribs.size();
+        ac.mvAtoms.push_back(at);
+    }
+    plan->wsElems = wsUsed();
+    plan->tileCount = ac.mvAtoms.size() + pr.mvProj.size();
+    plan->contribCount = ac.mvContribs.size();
+    return plan;
+}
+
 // ---------------------------------------------------------------------------
 // H*H product with randomized sum-sketch compression.
 // ---------------------------------------------------------------------------

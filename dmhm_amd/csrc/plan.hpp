@@ -72,7 +72,34 @@ struct RngTask {
     uint64_t stream;        // stream id mixed into the counter
 };
 
-enum PhaseKind { PH_TILE, PH_QR, PH_SVD, PH_RNG, PH_SCALE_C };
+// Streaming H*X for few right-hand sides (HBM-bound): projections
+// P = V^T X per (low-rank block, row chunk) by one wave each, then one
+// workgroup per output row atom summing dense-leaf and U*P contributions.
+struct MvProjItem {
+    int64_t voff;           // slot A offset of V(row0, 0) (ld = ldv)
+    int64_t xrow;           // first X row of the chunk
+    int64_t poff;           // workspace offset of P (cap x k, ld cap)
+    int32_t ldv, rows;      // V leading dimension, rows in this chunk
+    int32_t rankIdx, cap;
+    int32_t atomic;         // several chunks share P: accumulate atomically
+    int32_t pad;
+};
+struct MvAtom {
+    int64_t yrow;
+    int32_t m, cb, ce, pad;
+};
+struct MvContrib {
+    int64_t loff;           // slot A offset of the first row of L (dense leaf or U)
+    int64_t roff;           // X row offset (kind 0) or workspace offset of P (kind 1)
+    int32_t ld;             // leading dimension of L
+    int32_t q;              // inner dimension (static) ...
+    int32_t rankIdx;        // ... or the block's rank (kind 1)
+    int32_t kind;           // 0: dense leaf times X rows, 1: U rows times P
+    int32_t ldr;            // P leading dimension (cap) for kind 1
+    int32_t pad;
+};
+
+enum PhaseKind { PH_TILE, PH_QR, PH_SVD, PH_RNG, PH_SCALE_C, PH_MV_ZERO, PH_MV_PROJ, PH_MV_ACC };
 
 struct Phase {
     PhaseKind kind;
@@ -81,6 +108,12 @@ struct Phase {
     std::vector<QrTask> qr;
     std::vector<SvdTask> svd;
     std::vector<RngTask> rng;
+    std::vector<MvProjItem> mvProj;
+    std::vector<MvAtom> mvAtoms;
+    std::vector<MvContrib> mvContribs;
+    int64_t zeroOff = 0, zeroCount = 0;   // PH_MV_ZERO range in the workspace
+    int mvK = 0, ldx = 0, ldy = 0;
+    void* dAux = nullptr;
     int qrK = 0;            // panel width of the QR phase
     // device copies (filled by upload)
     void* dTiles = nullptr; void* dContribs = nullptr; void* dTasks = nullptr;
@@ -145,6 +178,8 @@ public:
 
     // Y (n x k, ld ldy) := alpha op(H) X + beta Y, H in SLOT_A with layout L.
     Plan* matvec(const Layout& L, bool trans, int k, int ldx, int ldy);
+    // Streaming form of the non-transposed product for k <= 8.
+    Plan* matvecStream(const Layout& L, int k, int ldx, int ldy);
 
     // C := alpha A B with randomized sum-sketch compression of every admissible
     // block (sketch width kS = maxRank + oversample), A/B/C layouts given.

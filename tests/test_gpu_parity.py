@@ -130,10 +130,11 @@ def test_whole_matrix_routines():
     np.testing.assert_allclose(F.to_dense(), D, rtol=1e-5, atol=1e-5 * np.abs(D).max())
 
 
-@pytest.mark.parametrize("name,n,L,rank,strong", [("c1-64", 64, 4, 8, True), ("16-weak", 16, 3, 4, False)])
+@pytest.mark.parametrize("name,n,L,rank,strong", [("c1-64", 64, 4, 8, True), ("16-weak-r8", 16, 3, 8, False)])
 def test_hh_sparse_operands_match_oracle(name, n, L, rank, strong):
-    """A*A of an imported stencil: exact in both implementations (admissible
-    products are exactly representable), so the outputs must agree."""
+    """A*A of an imported stencil: the admissible blocks of A^2 have rank
+    <= 2 x (interface width) <= maxRank here, so both implementations are exact
+    and must agree."""
     op = W.small_operator(n, L)
     H = dmhm_amd.HMat2d.from_csr(op["sizes"], L, rank, strong, *_args(op))
     R = orc.HMat.from_csr(2, op["sizes"], L, rank, strong, *_args(op))
@@ -141,6 +142,24 @@ def test_hh_sparse_operands_match_oracle(name, n, L, rank, strong):
     orc.rng_initialize()
     Cr = R.matmul(R, alpha=-1.0)
     assert _relmax(C.to_dense(), Cr.to_dense()) < 1e-12
+
+
+def test_hh_truncating_product_not_worse_than_reference_algorithm():
+    """Weak admissibility at rank 4: A^2's admissible blocks need more than
+    rank 4, so both implementations truncate. The sum-sketch compression must
+    be at least as accurate as the reference's sequential rounding (within
+    10%) against the exact product."""
+    op = W.small_operator(16, 3)
+    H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 4, False, *_args(op))
+    R = orc.HMat.from_csr(2, op["sizes"], 3, 4, False, *_args(op))
+    D = W.csr_to_dense(*_args(op), 256)
+    P = D @ D
+    C = H.Multiply(1.0, H)
+    orc.rng_initialize()
+    Cr = R.matmul(R)
+    e_gpu = np.linalg.norm(C.to_dense() - P, 2)
+    e_ref = np.linalg.norm(Cr.to_dense() - P, 2)
+    assert e_gpu <= 1.1 * e_ref
 
 
 @pytest.mark.parametrize("dim,n,L,rank,strong", [(2, 16, 3, 16, True), (2, 8, 3, 16, False), (3, 8, 3, 8, True)])
