@@ -24,22 +24,32 @@ namespace dmhm {
 
 // ---------------------------------------------------------------------------
 // Tile GEMM engine.
-// Workgroup = 256 threads = 4 waves. Thread (i = tid & 63, g = tid >> 6) owns
-// output rows i and columns j = g + 4*kk, kk < KPT. L (m x q) and R (q x k)
-// are staged through LDS in chunks of 64 along q; R is pre-scaled by alpha.
+// Workgroup = 256 threads = 4 waves. With MR = 64 rows, thread
+// (i = tid & 63, g = tid >> 6) owns row i and columns j = g + 4*kk; with
+// MR = 16 (small projection tiles) thread (i = tid & 15, g = tid >> 4) owns
+// row i and columns j = g + 16*kk. L (m x q) and R (q x k) are staged through
+// LDS in chunks of QC along q: all global loads of a chunk are issued before
+// any LDS store (latency overlap); R is pre-scaled by alpha.
 // ---------------------------------------------------------------------------
-template <typename T, int KPT, int ROLE>
+template <typename T> struct TileCfg { static constexpr int QC = 64; };
+template <> struct TileCfg<double> { static constexpr int QC = 32; };
+
+template <typename T, int KPT, int ROLE, int MR>
 __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restrict__ tiles,
                                                    const Contrib* __restrict__ cs, T alpha, T beta)
 {
-    __shared__ T Ls[64][65];
-    __shared__ T Rs[64][65];
+    constexpr int QC = TileCfg<T>::QC;
+    constexpr int GS = 256 / MR;                // column groups
+    constexpr int LN = (MR * QC + 255) / 256;   // L elements staged per thread
+    constexpr int RN = (QC * 64 + 255) / 256;   // R elements staged per thread
+    __shared__ T Ls[QC][MR + 1];
+    __shared__ T Rs[QC][65];
     const Tile tl = tiles[blockIdx.x];
     int m = tl.m, k = tl.k;
     if (tl.mrank >= 0) m = min(m, max(0, b.rank[tl.mslot][tl.mrank] - tl.moff));
     if (tl.krank >= 0) k = min(k, max(0, b.rank[tl.kslot][tl.krank] - tl.koff));
     if (m <= 0 || k <= 0) return;
-    const int tid = threadIdx.x, i = tid & 63, g = tid >> 6;
+    const int tid = threadIdx.x, i = tid % MR, g = tid / MR;
     T acc[KPT];
 #pragma unroll
     for (int kk = 0; kk < KPT; ++kk) acc[kk] = T(0);
@@ -53,35 +63,49 @@ __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restri
         const T* __restrict__ R = reinterpret_cast<const T*>(b.val[C.rslot]) + C.roff;
         const T a = (C.flags & CF_RUNTIME_ALPHA) ? alpha * T(C.alpha) : T(C.alpha);
         const bool tL = C.flags & CF_TRANS_L, tR = C.flags & CF_TRANS_R;
-        for (int p0 = 0; p0 < q; p0 += 64) {
-            const int qc = min(64, q - p0);
-            if (!tL) {
-                for (int e = tid; e < 64 * qc; e += 256) {
-                    const int ii = e & 63, pp = e >> 6;
-                    Ls[pp][ii] = (ii < m) ? L[ii + (int64_t)(p0 + pp) * C.ldl] : T(0);
-                }
-            } else {
-                for (int e = tid; e < 64 * 64; e += 256) {
-                    const int pp = e & 63, ii = e >> 6;
-                    if (pp < qc) Ls[pp][ii] = (ii < m) ? L[(p0 + pp) + (int64_t)ii * C.ldl] : T(0);
+        for (int p0 = 0; p0 < q; p0 += QC) {
+            const int qc = min(QC, q - p0);
+            T lr[LN], rr[RN];
+            // L(ii, pp): non-transposed walks ii fastest, transposed walks pp fastest.
+#pragma unroll
+            for (int u = 0; u < LN; ++u) {
+                const int e = tid + 256 * u;
+                int ii, pp;
+                if (!tL) { ii = e % MR; pp = e / MR; } else { pp = e % QC; ii = e / QC; }
+                const bool ok = e < MR * QC && ii < m && pp < qc;
+                lr[u] = ok ? (!tL ? L[ii + (int64_t)(p0 + pp) * C.ldl] : L[(p0 + pp) + (int64_t)ii * C.ldl]) : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < RN; ++u) {
+                const int e = tid + 256 * u;
+                int jj, pp;
+                if (!tR) { pp = e % QC; jj = e / QC; } else { jj = e % 64; pp = e / 64; }
+                const bool ok = e < QC * 64 && jj < k && pp < qc;
+                rr[u] = ok ? (!tR ? R[(p0 + pp) + (int64_t)jj * C.ldr] : R[jj + (int64_t)(p0 + pp) * C.ldr]) : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < LN; ++u) {
+                const int e = tid + 256 * u;
+                if (e < MR * QC) {
+                    int ii, pp;
+                    if (!tL) { ii = e % MR; pp = e / MR; } else { pp = e % QC; ii = e / QC; }
+                    Ls[pp][ii] = lr[u];
                 }
             }
-            if (!tR) {
-                for (int e = tid; e < 64 * 64; e += 256) {
-                    const int pp = e & 63, jj = e >> 6;
-                    if (pp < qc) Rs[pp][jj] = (jj < k) ? a * R[(p0 + pp) + (int64_t)jj * C.ldr] : T(0);
-                }
-            } else {
-                for (int e = tid; e < 64 * qc; e += 256) {
-                    const int jj = e & 63, pp = e >> 6;
-                    Rs[pp][jj] = (jj < k) ? a * R[jj + (int64_t)(p0 + pp) * C.ldr] : T(0);
+#pragma unroll
+            for (int u = 0; u < RN; ++u) {
+                const int e = tid + 256 * u;
+                if (e < QC * 64) {
+                    int jj, pp;
+                    if (!tR) { pp = e % QC; jj = e / QC; } else { jj = e % 64; pp = e / 64; }
+                    Rs[pp][jj] = a * rr[u];
                 }
             }
             __syncthreads();
             for (int p = 0; p < qc; ++p) {
                 const T l = Ls[p][i];
 #pragma unroll
-                for (int kk = 0; kk < KPT; ++kk) acc[kk] += l * Rs[p][g + 4 * kk];
+                for (int kk = 0; kk < KPT; ++kk) acc[kk] += l * Rs[p][g + GS * kk];
             }
             __syncthreads();
         }
@@ -92,7 +116,7 @@ __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restri
         const bool accum = (tl.flags & TF_ACCUMULATE);
 #pragma unroll
         for (int kk = 0; kk < KPT; ++kk) {
-            const int j = g + 4 * kk;
+            const int j = g + GS * kk;
             if (j < k) {
                 T* o = out + i + (int64_t)j * tl.ld;
                 if (useBeta) *o = beta * *o + acc[kk];
@@ -103,6 +127,19 @@ __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restri
     }
 }
 
+template <typename T, int ROLE, int MR>
+static void launch_tile_mr(int n, int kmax, const Bases& b, const Tile* t, const Contrib* c, T alpha, T beta,
+                           hipStream_t s)
+{
+    constexpr int GS = 256 / MR;
+    const int kpt = (kmax + GS - 1) / GS;
+    if (kpt <= 1) tile_kernel<T, 1, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else if (kpt <= 2) tile_kernel<T, 2, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else if (kpt <= 4) tile_kernel<T, 4, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else if (kpt <= 8) tile_kernel<T, 8, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else tile_kernel<T, 16, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+}
+
 template <typename T>
 void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s)
 {
@@ -111,19 +148,14 @@ void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStre
     const Tile* t = static_cast<const Tile*>(ph.dTiles);
     const Contrib* c = static_cast<const Contrib*>(ph.dContribs);
     const int kmax = ph.maxKTile;
-    // ROLE only distinguishes projection and accumulation launches in profiles.
-    if (ph.role == 1) {
-        if (kmax <= 4) tile_kernel<T, 1, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-        else if (kmax <= 8) tile_kernel<T, 2, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-        else if (kmax <= 16) tile_kernel<T, 4, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-        else if (kmax <= 32) tile_kernel<T, 8, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-        else tile_kernel<T, 16, 1><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    // ROLE only distinguishes projection and accumulation launches in profiles;
+    // MR = 16 serves phases whose tiles all have at most 16 rows.
+    if (ph.maxMTile <= 16) {
+        if (ph.role == 1) launch_tile_mr<T, 1, 16>(n, kmax, b, t, c, alpha, beta, s);
+        else launch_tile_mr<T, 0, 16>(n, kmax, b, t, c, alpha, beta, s);
     } else {
-        if (kmax <= 4) tile_kernel<T, 1, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-        else if (kmax <= 8) tile_kernel<T, 2, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-        else if (kmax <= 16) tile_kernel<T, 4, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-        else if (kmax <= 32) tile_kernel<T, 8, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-        else tile_kernel<T, 16, 0><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+        if (ph.role == 1) launch_tile_mr<T, 1, 64>(n, kmax, b, t, c, alpha, beta, s);
+        else launch_tile_mr<T, 0, 64>(n, kmax, b, t, c, alpha, beta, s);
     }
     HIP_LAUNCH_CHECK();
 }

@@ -20,6 +20,7 @@ void PhaseBuilder::finalize()
         t.cb = base + count[i];
         t.ce = base + count[i + 1];
         ph_.maxKTile = std::max(ph_.maxKTile, t.k);
+        ph_.maxMTile = std::max(ph_.maxMTile, t.m);
         for (int c = t.cb; c < t.ce; ++c) ph_.flops += 2.0 * t.m * ph_.contribs[c].q * t.k;
     }
     pending_.clear();
@@ -354,7 +355,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     plan->phases.reserve(16);
 
     // Sketch panels per admissible C block.
-    struct Sk { int b; int64_t omega, Y, Z, R, G1, G2; int m, n; };
+    struct Sk { int b; int64_t omega, Y, Z, R, G1, G2, D; int m, n; bool leaf; };
     std::vector<Sk> sk;
     std::unordered_map<uint64_t, int64_t> omegaOf;     // (level, s) -> panel
     std::unordered_map<uint64_t, int64_t> t1Of;        // B block -> T1 panel
@@ -366,6 +367,9 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         const Box& sb = g.boxes[C.level][C.s];
         Sk s;
         s.b = lbk; s.m = tb.size; s.n = sb.size;
+        // Leaf-level admissible blocks are accumulated densely, then compressed.
+        s.leaf = (C.level == g.numLevels - 1);
+        s.D = s.leaf ? wsAlloc((int64_t)tb.size * sb.size) : -1;
         auto it = omegaOf.find(key2(C.level, C.s));
         if (it == omegaOf.end()) {
             const int64_t o = wsAlloc((int64_t)sb.size * kS);
@@ -409,7 +413,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         }
         for (const auto& s : sk)
             for (const auto& tm : nnTerms[s.b]) {
-                if (t1Of.count((uint64_t)tm.second)) continue;
+                if (s.leaf || t1Of.count((uint64_t)tm.second)) continue;
                 const Block& B = g.blocks[tm.second];
                 const Box& ub = g.boxes[B.level][B.t];
                 const int64_t T1 = wsAlloc((int64_t)ub.size * kS);
@@ -447,8 +451,54 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         plan->flops += 2.0 * bx.size * w.cap * kS;
     };
 
-    // 3. Y = T Omega for admissible C blocks; dense C leaves.
-    std::vector<std::vector<int64_t>> yProj(sk.size());
+    // Dense accumulation of a leaf-level C block: out (|t| x |s|) =
+    // sum D_A D_B + sum PU PV^T over its dense*dense and low-rank terms.
+    auto denseAccumulate = [&](PhaseBuilder& acc, int b, const Operand& out,
+                               const std::vector<std::pair<int, int>>& dd, uint8_t cflags) {
+        const Block& C = g.blocks[b];
+        const Box& tb = g.boxes[C.level][C.t];
+        const Box& sb = g.boxes[C.level][C.s];
+        PanelTiles pt = addPanelTiles(acc, out, C.level, C.t, sb.size, -1, 0, 0);
+        for (const auto& tm : dd) {
+            const Block& A = g.blocks[tm.first];
+            const Block& B = g.blocks[tm.second];
+            const int usz = g.boxes[A.level][A.s].size;
+            for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
+                const int ra = g.atomStart[a] - tb.off;
+                for (int c = 0; c < pt.kChunks; ++c)
+                    acc.add(pt.tileOfAtom[(size_t)(a - tb.atomBegin) * pt.kChunks + c],
+                            mkc(SLOT_A, LA.denseOff[A.denseIndex] + ra, tb.size, SLOT_B,
+                                LB.denseOff[B.denseIndex] + (int64_t)c * 64 * usz, usz, usz, -1, 0, cflags, 1.0f));
+            }
+            plan->flops += 2.0 * tb.size * usz * sb.size;
+        }
+        for (int wi : leafW[b]) {
+            const WFactor& w = W[wi];
+            const Operand pu = slicePU(w, tb), pv = slicePV(w, sb);
+            for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
+                const int ra = g.atomStart[a] - tb.off;
+                for (int c = 0; c < pt.kChunks; ++c)
+                    acc.add(pt.tileOfAtom[(size_t)(a - tb.atomBegin) * pt.kChunks + c],
+                            mkc(pu.slot, pu.off + ra, pu.ld, pv.slot, pv.off + 64 * c, pv.ld, w.cap, w.rankIndex,
+                                w.rankSlot, CF_TRANS_R | cflags, 1.0f));
+            }
+            plan->flops += 2.0 * tb.size * w.cap * sb.size;
+        }
+    };
+    {
+        Phase& ad = newPhase(PH_TILE);
+        PhaseBuilder acc(ad);
+        for (int bd : g.denseBlocks) {
+            const Block& C = g.blocks[bd];
+            denseAccumulate(acc, bd, {SLOT_C, LC.denseOff[C.denseIndex], g.boxes[C.level][C.t].size},
+                            ddTerms[bd], CF_RUNTIME_ALPHA);
+        }
+        for (const auto& s : sk)
+            if (s.leaf) denseAccumulate(acc, s.b, {SLOT_WS, s.D, s.m}, nnTerms[s.b], 0);
+        acc.finalize();
+    }
+
+    // 3. Y = T Omega for admissible C blocks.
     {
         Phase& p2 = newPhase(PH_TILE);
         p2.role = 1;
@@ -461,6 +511,15 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             const Box& sb = g.boxes[C.level][C.s];
             PanelTiles pt = addPanelTiles(acc, {SLOT_WS, s.Y, s.m}, C.level, C.t, kS, -1, 0, 0);
             const Operand om{SLOT_WS, s.omega, s.n};
+            if (s.leaf) {
+                for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
+                    const int ra = g.atomStart[a] - tb.off;
+                    acc.add(pt.tileOfAtom[a - tb.atomBegin],
+                            mkc(SLOT_WS, s.D + ra, s.m, SLOT_WS, s.omega, s.n, s.n, -1, 0, 0, 1.0f));
+                }
+                plan->flops += 2.0 * s.m * s.n * kS;
+                continue;
+            }
             for (int wi : leafW[s.b]) {
                 const WFactor& w = W[wi];
                 const int64_t P = wsAlloc((int64_t)w.cap * kS);
@@ -479,40 +538,6 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 const Operand in{SLOT_WS, t1Of[(uint64_t)tm.second], g.boxes[B.level][B.t].size};
                 subh(proj, acc, SLOT_A, LA, tm.first, false, in, B.level, B.t, pt, kS, -1, 0, 1.0f, 0,
                      &plan->flops);
-            }
-        }
-        // Dense C leaves: C = alpha (sum D_A D_B + sum PU PV^T).
-        for (int bd : g.denseBlocks) {
-            const Block& C = g.blocks[bd];
-            const Box& tb = g.boxes[C.level][C.t];
-            const Box& sb = g.boxes[C.level][C.s];
-            const Operand out{SLOT_C, LC.denseOff[C.denseIndex], tb.size};
-            PanelTiles pt = addPanelTiles(acc, out, C.level, C.t, sb.size, -1, 0, 0);
-            for (const auto& tm : ddTerms[bd]) {
-                const Block& A = g.blocks[tm.first];
-                const Block& B = g.blocks[tm.second];
-                const int usz = g.boxes[A.level][A.s].size;
-                for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
-                    const int ra = g.atomStart[a] - tb.off;
-                    for (int c = 0; c < pt.kChunks; ++c)
-                        acc.add(pt.tileOfAtom[(size_t)(a - tb.atomBegin) * pt.kChunks + c],
-                                mkc(SLOT_A, LA.denseOff[A.denseIndex] + ra, tb.size, SLOT_B,
-                                    LB.denseOff[B.denseIndex] + (int64_t)c * 64 * usz, usz, usz, -1, 0,
-                                    CF_RUNTIME_ALPHA, 1.0f));
-                }
-                plan->flops += 2.0 * tb.size * usz * sb.size;
-            }
-            for (int wi : leafW[bd]) {
-                const WFactor& w = W[wi];
-                const Operand pu = slicePU(w, tb), pv = slicePV(w, sb);
-                for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
-                    const int ra = g.atomStart[a] - tb.off;
-                    for (int c = 0; c < pt.kChunks; ++c)
-                        acc.add(pt.tileOfAtom[(size_t)(a - tb.atomBegin) * pt.kChunks + c],
-                                mkc(pu.slot, pu.off + ra, pu.ld, pv.slot, pv.off + 64 * c, pv.ld, w.cap,
-                                    w.rankIndex, w.rankSlot, CF_TRANS_R | CF_RUNTIME_ALPHA, 1.0f));
-                }
-                plan->flops += 2.0 * tb.size * w.cap * sb.size;
             }
         }
         proj.finalize();
@@ -547,6 +572,17 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             const Box& sb = g.boxes[C.level][C.s];
             PanelTiles zt = addPanelTiles(acc4, {SLOT_WS, s.Z, s.n}, C.level, C.s, kS, -1, 0, 0);
             const Operand Q{SLOT_WS, s.Y, s.m};
+            if (s.leaf) {
+                // Z = D^T Q: L(i, p) = D(p, i)
+                for (int a = sb.atomBegin; a < sb.atomEnd; ++a) {
+                    const int ra = g.atomStart[a] - sb.off;
+                    acc4.add(zt.tileOfAtom[a - sb.atomBegin],
+                             mkc(SLOT_WS, s.D + (int64_t)ra * s.m, s.m, SLOT_WS, s.Y, s.m, s.m, -1, 0, CF_TRANS_L,
+                                 1.0f));
+                }
+                plan->flops += 2.0 * s.m * s.n * kS;
+                continue;
+            }
             for (int wi : leafW[s.b]) {
                 const WFactor& w = W[wi];
                 const int64_t P = wsAlloc((int64_t)w.cap * kS);

@@ -118,6 +118,7 @@ struct Phase {
     // device copies (filled by upload)
     void* dTiles = nullptr; void* dContribs = nullptr; void* dTasks = nullptr;
     int maxKTile = 0;       // widest tile (selects the kernel variant)
+    int maxMTile = 0;       // tallest tile
     int role = 0;           // 0 accumulation, 1 projection (profiling identity)
     double flops = 0;       // algorithmic flops (2 m q k per contribution at static sizes)
 };
