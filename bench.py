@@ -49,6 +49,21 @@ def residual_power_estimate(A, X, n, steps=12, seed=3):
     return est
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a streaming
+    kernel of known size), or None if absent."""
+    path = os.path.join(ROOT, "profiles", "matvec_256sq_lognormal_fp32_k1_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(cfg, t_gpu_iters):
     """FP64 oracle (the reference algorithm restated in C++) on one host core,
     on a bounded sample of the same workload: the first Schulz product
@@ -238,9 +253,10 @@ def main():
                    "f32_k64_ms": ms64[0], "f32_k64_GBps": gbs64, "f32_k64_TFLOPs": tflops64,
                    "f64_k1_ms": msd[0], "f64_k1_GBps": gbsd, "f32_k1_bytes": by1.sum()},
         "hh_product": hh,
-        "roofline": {"bound": "hbm", "kernel": "tile_kernel<float,1,0> (H*x accumulation launch, FP32, k=1)",
+        "roofline": {"bound": "hbm", "kernel": "mv_acc_kernel<float,1> (H*x accumulation launch, FP32, k=1)",
                      "achieved": acc_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": acc_gbs / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": by1[1], "launch_ms": ms1[2]},
+                     "traffic": measured_traffic("mv_acc_kernel<float,1>"), "algorithmic_bytes": by1[1],
+                     "launch_ms": ms1[2]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -6,5 +6,7 @@ the reference interface plus the synthetic workload generators.
 """
 from ._lib import (DMHM_F64, DMHM_F32, DmhmError, NonConformalError, NotConvergedError,  # noqa: F401
                    init, load)
-from .hmat import HMat2d, HMat3d, EstimateTwoNorm, build_natural_to_hierarchical_map, geometry_block_counts, geometry_product_terms, geometry_leaves  # noqa: F401
+from .hmat import (HMat2d, HMat3d, EstimateTwoNorm, RoundedUpdate, Compress,  # noqa: F401
+                   build_natural_to_hierarchical_map, geometry_block_counts, geometry_product_terms,
+                   geometry_leaves)
 from . import workloads  # noqa: F401

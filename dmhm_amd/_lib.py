@@ -90,6 +90,14 @@ SIGNATURES = {
                                                 ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
                                                 ctypes.c_int, ctypes.c_int]),
     "dmhm_hmat_multiply_hmat": (ctypes.c_int, [ctypes.c_double, _h, _h, _hp, ctypes.POINTER(HHOpts)]),
+    "dmhm_hmat_multiply_hmat_update": (ctypes.c_int, [ctypes.c_double, _h, _h, ctypes.c_double, _h,
+                                                      ctypes.POINTER(HHOpts)]),
+    "dmhm_hmat_update_with": (ctypes.c_int, [_h, ctypes.c_double, _h, ctypes.POINTER(HHOpts)]),
+    "dmhm_hmat_create_from_lowrank": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_int, _dp, _dp, _hp]),
+    "dmhm_rounded_update": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
+    "dmhm_compress_dense": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip]),
     "dmhm_hmat_copy": (ctypes.c_int, [_h, _hp]),
     "dmhm_hmat_transpose": (ctypes.c_int, [_h, _hp]),
     "dmhm_hmat_scale": (ctypes.c_int, [_h, ctypes.c_double]),

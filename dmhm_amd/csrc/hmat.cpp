@@ -239,6 +239,252 @@ static dmhm_hmat_s* multiply_hmat(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, 
     return C.release();
 }
 
+static dmhm_hmat_s* copy_hmat(dmhm_hmat_s* s, int prec);
+
+static void multiply_hmat_update(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, double beta, dmhm_hmat_s* C,
+                                 const dmhm_hh_opts& o)
+{
+    if (A->gc != B->gc || A->gc != C->gc) throw Error(DMHM_ENONCONF, "Attempted nonconformal matrix-matrix multiply");
+    if (A->prec != B->prec || A->prec != C->prec) throw Error(DMHM_EINVAL, "operands must share a precision");
+    GeoCtx& gc = *A->gc;
+    const int maxRank = C->maxRank;
+    for (int c : C->lay->cap)
+        if (c > 64) throw Error(DMHM_EINVAL, "low-rank capacity above 64 in a product");
+    std::unique_ptr<dmhm_hmat_s> Cold(copy_hmat(C, C->prec));
+    const int over = o.oversample > 0 ? o.oversample : 4;
+    const std::string key = "hhu:" + std::to_string(A->lay->id) + ":" + std::to_string(B->lay->id) + ":" +
+                            std::to_string(C->lay->id) + ":" + std::to_string(maxRank) + ":" + std::to_string(over);
+    Plan& p = get_plan(gc, key, [&] {
+        Planner pl(gc.geo);
+        return pl.hhProduct(*A->lay, *B->lay, *C->lay, maxRank, over, Planner::HH_UPDATE);
+    });
+    ensure_ws(p, A->esize());
+    Bases b = bases_for(A, B, C, Cold->vals, nullptr);
+    b.rank[SLOT_X] = Cold->ranks;
+    TruncPolicy pol{o.tolMode, o.relTol};
+    if (A->prec == DMHM_F64) run_plan<double>(p, b, alpha, beta, pol, o.seed);
+    else run_plan<float>(p, b, (float)alpha, (float)beta, pol, o.seed);
+    HIPCHECK(hipStreamSynchronize(g_lib.stream));
+}
+
+static void update_with(dmhm_hmat_s* A, double alpha, dmhm_hmat_s* B, const dmhm_hh_opts& o)
+{
+    if (A->gc != B->gc) throw Error(DMHM_ENONCONF, "Attempted nonconformal update");
+    if (A->prec != B->prec) throw Error(DMHM_EINVAL, "operands must share a precision");
+    GeoCtx& gc = *A->gc;
+    const int maxRank = A->maxRank;
+    auto LC = std_layout(gc, maxRank);
+    std::unique_ptr<dmhm_hmat_s> C(alloc_hmat(A->gc, LC, A->prec, maxRank));
+    const int over = o.oversample > 0 ? o.oversample : 4;
+    const std::string key = "hha:" + std::to_string(A->lay->id) + ":" + std::to_string(B->lay->id) + ":" +
+                            std::to_string(LC->id) + ":" + std::to_string(maxRank) + ":" + std::to_string(over);
+    Plan& p = get_plan(gc, key, [&] { Planner pl(gc.geo); return pl.hhAdd(*A->lay, *B->lay, *LC, maxRank, over); });
+    ensure_ws(p, A->esize());
+    Bases b = bases_for(A, B, C.get(), nullptr, nullptr);
+    TruncPolicy pol{o.tolMode, o.relTol};
+    const int64_t nd = A->lay->denseElems;
+    if (A->prec == DMHM_F64) {
+        run_plan<double>(p, b, alpha, 0.0, pol, o.seed);
+        launch_axpby<double>((const double*)A->vals, (const double*)B->vals, alpha, (double*)C->vals, nd, g_lib.stream);
+    } else {
+        run_plan<float>(p, b, (float)alpha, 0.0f, pol, o.seed);
+        launch_axpby<float>((const float*)A->vals, (const float*)B->vals, (float)alpha, (float*)C->vals, nd, g_lib.stream);
+    }
+    HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    std::swap(A->lay, C->lay);
+    std::swap(A->vals, C->vals);
+    std::swap(A->ranks, C->ranks);
+}
+
+// One-off device runs of the QR / SVD / tile kernels on host data (FP64).
+struct OneOff {
+    Plan plan;
+    DeviceBuffer ws;
+    int* rank = nullptr;
+    ~OneOff() { ws.release(); if (rank) (void)hipFree(rank); }
+    void run() {
+        plan.upload();
+        Bases b{};
+        b.val[SLOT_WS] = ws.ptr;
+        b.rank[SLOT_C] = rank;
+        run_plan<double>(plan, b, 1.0, 0.0, {0, 0}, 0);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    }
+};
+
+// Tiles computing out (rows x k, ld rows) = L (rows x q, ld ldl) * R (q x k, ld ldr), all in the workspace.
+static void add_gemm_tiles(Phase& ph, int64_t out, int rows, int k, int64_t L, int ldl, int q, int64_t R, int ldr,
+                           uint8_t rflags)
+{
+    PhaseBuilder pb(ph);
+    for (int r0 = 0; r0 < rows; r0 += 64)
+        for (int c0 = 0; c0 < k; c0 += 64) {
+            Tile t{};
+            t.off = out + r0 + (int64_t)c0 * rows; t.ld = rows;
+            t.m = std::min(64, rows - r0); t.k = std::min(64, k - c0);
+            t.mrank = t.krank = -1; t.slot = SLOT_WS;
+            const int id = pb.addTile(t);
+            Contrib c{};
+            c.loff = L + r0; c.ldl = ldl; c.q = q; c.qrank = -1;
+            c.roff = (rflags & CF_TRANS_R) ? R + c0 : R + (int64_t)c0 * ldr; c.ldr = ldr;
+            c.lslot = SLOT_WS; c.rslot = SLOT_WS; c.flags = rflags; c.alpha = 1.0f;
+            pb.add(id, c);
+        }
+    pb.finalize();
+}
+
+static void rounded_update_dev(int maxRank, double alpha, int m, int n, int rA, const double* AU, const double* AV,
+                               double beta, int rB, const double* BU, const double* BV, double* outU, double* outV,
+                               int* outRank)
+{
+    const int r = rA + rB;
+    const int cap = std::min(std::min(r, std::min(m, n)), maxRank);
+    if (cap == r) {
+        // Early exit: exact concatenation [beta BU, alpha AU], [BV, AV] (RoundedUpdate.cpp:34-53).
+        for (int j = 0; j < rB; ++j) {
+            for (int i = 0; i < m; ++i) outU[i + (size_t)j * m] = beta * BU[i + (size_t)j * m];
+            for (int i = 0; i < n; ++i) outV[i + (size_t)j * n] = BV[i + (size_t)j * n];
+        }
+        for (int j = 0; j < rA; ++j) {
+            for (int i = 0; i < m; ++i) outU[i + (size_t)(rB + j) * m] = alpha * AU[i + (size_t)j * m];
+            for (int i = 0; i < n; ++i) outV[i + (size_t)(rB + j) * n] = AV[i + (size_t)j * n];
+        }
+        *outRank = r;
+        return;
+    }
+    if (r > 64) throw Error(DMHM_EINVAL, "rounded update supports rA + rB <= 64");
+    // Panels with fewer rows than columns are padded with zero rows so the
+    // panel QR sees at least r rows; the padded rows of the result are dropped.
+    const int mp = std::max(m, r), np = std::max(n, r);
+    OneOff job;
+    int64_t o = 0;
+    auto take = [&](int64_t e) { const int64_t x = o; o += (e + 63) & ~int64_t(63); return x; };
+    const int64_t Uc = take((int64_t)mp * r), Vc = take((int64_t)np * r), R1 = take(r * r), R2 = take(r * r),
+                  Wm = take(r * r), G1 = take(r * r), G2 = take(r * r), oU = take((int64_t)mp * r),
+                  oV = take((int64_t)np * r);
+    std::vector<double> host(o, 0.0);
+    for (int j = 0; j < rA; ++j) {
+        for (int i = 0; i < m; ++i) host[Uc + i + (size_t)j * mp] = alpha * AU[i + (size_t)j * m];
+        for (int i = 0; i < n; ++i) host[Vc + i + (size_t)j * np] = AV[i + (size_t)j * n];
+    }
+    for (int j = 0; j < rB; ++j) {
+        for (int i = 0; i < m; ++i) host[Uc + i + (size_t)(rA + j) * mp] = beta * BU[i + (size_t)j * m];
+        for (int i = 0; i < n; ++i) host[Vc + i + (size_t)(rA + j) * np] = BV[i + (size_t)j * n];
+    }
+    job.ws.ensure(o * sizeof(double));
+    HIPCHECK(hipMemcpy(job.ws.ptr, host.data(), o * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMalloc((void**)&job.rank, sizeof(int)));
+    job.plan.phases.resize(5);
+    Phase& q = job.plan.phases[0];
+    q.kind = PH_QR;
+    q.qr.push_back(QrTask{Uc, R1, mp, r, SLOT_WS, 0, 0, 0, 0});
+    q.qr.push_back(QrTask{Vc, R2, np, r, SLOT_WS, 0, 0, 0, 0});
+    job.plan.phases[1].kind = PH_TILE;                        // W = R1 R2^T
+    add_gemm_tiles(job.plan.phases[1], Wm, r, r, R1, r, r, R2, r, CF_TRANS_R);
+    Phase& sv = job.plan.phases[2];
+    sv.kind = PH_SVD;
+    sv.svd.push_back(SvdTask{Wm, G1, G2, r, r, 0, cap});      // W = U S V^T: G1 = V S, G2 = U
+    job.plan.phases[3].kind = PH_TILE;                        // out U = Q1 U_W
+    add_gemm_tiles(job.plan.phases[3], oU, mp, r, Uc, mp, r, G2, r, 0);
+    job.plan.phases[4].kind = PH_TILE;                        // out V = Q2 V_W S
+    add_gemm_tiles(job.plan.phases[4], oV, np, r, Vc, np, r, G1, r, 0);
+    job.run();
+    int keep = 0;
+    HIPCHECK(hipMemcpy(&keep, job.rank, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(host.data(), job.ws.ptr, o * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < keep; ++j) {
+        for (int i = 0; i < m; ++i) outU[i + (size_t)j * m] = host[oU + i + (size_t)j * mp];
+        for (int i = 0; i < n; ++i) outV[i + (size_t)j * n] = host[oV + i + (size_t)j * np];
+    }
+    *outRank = keep;
+}
+
+static void compress_dense_dev(int maxRank, int m, int n, const double* D, double* outU, double* outV, int* outRank)
+{
+    if (std::min(m, n) > 64) throw Error(DMHM_EINVAL, "dense compression supports min(m, n) <= 64");
+    if (n > 64) {
+        // Compress D^T and swap the factors.
+        std::vector<double> Dt((size_t)m * n);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < m; ++i) Dt[j + (size_t)i * n] = D[i + (size_t)j * m];
+        compress_dense_dev(maxRank, n, m, Dt.data(), outV, outU, outRank);
+        return;
+    }
+    const int k = n, cap = std::min(std::min(m, n), maxRank);
+    const int mp = std::max(m, k);                            // zero rows pad a wide panel
+    OneOff job;
+    int64_t o = 0;
+    auto take = [&](int64_t e) { const int64_t x = o; o += (e + 63) & ~int64_t(63); return x; };
+    const int64_t Dd = take((int64_t)mp * n), R = take(k * k), G1 = take(k * k), G2 = take(k * k),
+                  oU = take((int64_t)mp * k);
+    std::vector<double> host(o, 0.0);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) host[Dd + i + (size_t)j * mp] = D[i + (size_t)j * m];
+    job.ws.ensure(o * sizeof(double));
+    HIPCHECK(hipMemcpy(job.ws.ptr, host.data(), o * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMalloc((void**)&job.rank, sizeof(int)));
+    job.plan.phases.resize(3);
+    job.plan.phases[0].kind = PH_QR;
+    job.plan.phases[0].qr.push_back(QrTask{Dd, R, mp, k, SLOT_WS, 0, 0, 0, 0});
+    job.plan.phases[1].kind = PH_SVD;                         // R = U S V^T: G1 = V S, G2 = U
+    job.plan.phases[1].svd.push_back(SvdTask{R, G1, G2, k, k, 0, cap});
+    job.plan.phases[2].kind = PH_TILE;                        // out U = Q U_R
+    add_gemm_tiles(job.plan.phases[2], oU, mp, k, Dd, mp, k, G2, k, 0);
+    job.run();
+    int keep = 0;
+    HIPCHECK(hipMemcpy(&keep, job.rank, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(host.data(), job.ws.ptr, o * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < keep; ++j) {
+        for (int i = 0; i < m; ++i) outU[i + (size_t)j * m] = host[oU + i + (size_t)j * mp];
+        for (int i = 0; i < n; ++i) outV[i + (size_t)j * n] = host[G1 + i + (size_t)j * k];
+    }
+    *outRank = keep;
+}
+
+static dmhm_hmat_s* create_from_lowrank(int dim, const int* sizes, int L, int maxRank, bool strong, int prec, int r,
+                                        const double* U, const double* V)
+{
+    auto gc = get_geo(dim, sizes, L, strong);
+    const Geometry& g = gc->geo;
+    std::vector<int> caps(g.lrBlocks.size(), std::max(std::max(maxRank, 1), r));
+    std::shared_ptr<Layout> lay;
+    if (r <= std::max(maxRank, 1)) lay = std_layout(*gc, maxRank);
+    else {
+        std::lock_guard<std::mutex> lk(g_lib.mu);
+        lay = std::make_shared<Layout>(Layout::make(g, caps, g_lib.nextLayoutId++));
+    }
+    std::unique_ptr<dmhm_hmat_s> h(alloc_hmat(gc, lay, prec, maxRank));
+    std::vector<double> host(lay->totalElems(), 0.0);
+    const int n = g.n;
+    for (size_t i = 0; i < g.denseBlocks.size(); ++i) {
+        const int b = g.denseBlocks[i];
+        const Box& tb = g.tbox(b); const Box& sb = g.sbox(b);
+        double* Dp = host.data() + lay->denseOff[i];
+        for (int c = 0; c < sb.size; ++c)
+            for (int rr = 0; rr < tb.size; ++rr) {
+                double sum = 0;
+                for (int p = 0; p < r; ++p) sum += U[(tb.off + rr) + (size_t)p * n] * V[(sb.off + c) + (size_t)p * n];
+                Dp[rr + (size_t)c * tb.size] = sum;
+            }
+    }
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        const int b = g.lrBlocks[i];
+        const Box& tb = g.tbox(b); const Box& sb = g.sbox(b);
+        for (int p = 0; p < r; ++p) {
+            for (int rr = 0; rr < tb.size; ++rr) host[lay->uOff[i] + rr + (size_t)p * tb.size] = U[(tb.off + rr) + (size_t)p * n];
+            for (int rr = 0; rr < sb.size; ++rr) host[lay->vOff[i] + rr + (size_t)p * sb.size] = V[(sb.off + rr) + (size_t)p * n];
+        }
+    }
+    if (prec == DMHM_F64) HIPCHECK(hipMemcpy(h->vals, host.data(), host.size() * 8, hipMemcpyHostToDevice));
+    else {
+        std::vector<float> hf(host.begin(), host.end());
+        HIPCHECK(hipMemcpy(h->vals, hf.data(), hf.size() * 4, hipMemcpyHostToDevice));
+    }
+    std::vector<int> ranks(g.lrBlocks.size(), r);
+    if (!ranks.empty()) HIPCHECK(hipMemcpy(h->ranks, ranks.data(), ranks.size() * sizeof(int), hipMemcpyHostToDevice));
+    return h.release();
+}
+
 static dmhm_hmat_s* copy_hmat(dmhm_hmat_s* s, int prec)
 {
     std::unique_ptr<dmhm_hmat_s> d(alloc_hmat(s->gc, s->lay, prec, s->maxRank));
@@ -985,6 +1231,65 @@ int dmhm_hmat_multiply_hmat(double alpha, dmhm_hmat A, dmhm_hmat B, dmhm_hmat* C
         if (opts) o = *opts;
         *C = multiply_hmat(alpha, A, B, o);
         HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_hmat_multiply_hmat_update(double alpha, dmhm_hmat A, dmhm_hmat B, double beta, dmhm_hmat C,
+                                   const dmhm_hh_opts* opts)
+{
+    return guard([&] {
+        require_ready();
+        if (!A || !B || !C) throw Error(DMHM_EINVAL, "null pointer");
+        dmhm_hh_opts o{4, 0, 1e-10, 1};
+        if (opts) o = *opts;
+        multiply_hmat_update(alpha, A, B, beta, C, o);
+    });
+}
+
+int dmhm_hmat_update_with(dmhm_hmat A, double alpha, dmhm_hmat B, const dmhm_hh_opts* opts)
+{
+    return guard([&] {
+        require_ready();
+        if (!A || !B) throw Error(DMHM_EINVAL, "null pointer");
+        dmhm_hh_opts o{4, 0, 1e-10, 1};
+        if (opts) o = *opts;
+        update_with(A, alpha, B, o);
+    });
+}
+
+int dmhm_hmat_create_from_lowrank(int dim, const int* sizes, int numLevels, int maxRank, int strong, int precision,
+                                  int r, const double* U, const double* V, dmhm_hmat* out)
+{
+    return guard([&] {
+        require_ready();
+        if (!sizes || !out || (r > 0 && (!U || !V))) throw Error(DMHM_EINVAL, "null pointer");
+        if (precision != DMHM_F64 && precision != DMHM_F32) throw Error(DMHM_EINVAL, "precision must be 8 or 4");
+        if (maxRank < 0 || maxRank > 64 || r < 0) throw Error(DMHM_EINVAL, "bad rank");
+        *out = create_from_lowrank(dim, sizes, numLevels, maxRank, strong != 0, precision, r, U, V);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_rounded_update(int maxRank, double alpha, int m, int n, int rA, const double* AU, const double* AV,
+                        double beta, int rB, const double* BU, const double* BV, double* outU, double* outV,
+                        int* outRank)
+{
+    return guard([&] {
+        require_ready();
+        if (m <= 0 || n <= 0 || rA < 0 || rB < 0 || maxRank < 0) throw Error(DMHM_EINVAL, "bad sizes");
+        if (!outU || !outV || !outRank || (rA && (!AU || !AV)) || (rB && (!BU || !BV)))
+            throw Error(DMHM_EINVAL, "null pointer");
+        rounded_update_dev(maxRank, alpha, m, n, rA, AU, AV, beta, rB, BU, BV, outU, outV, outRank);
+    });
+}
+
+int dmhm_compress_dense(int maxRank, int m, int n, const double* D, double* outU, double* outV, int* outRank)
+{
+    return guard([&] {
+        require_ready();
+        if (m <= 0 || n <= 0 || maxRank < 0) throw Error(DMHM_EINVAL, "bad sizes");
+        if (!D || !outU || !outV || !outRank) throw Error(DMHM_EINVAL, "null pointer");
+        compress_dense_dev(maxRank, m, n, D, outU, outV, outRank);
     });
 }
 

@@ -61,7 +61,8 @@ __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restri
         if (q <= 0) continue;
         const T* __restrict__ L = reinterpret_cast<const T*>(b.val[C.lslot]) + C.loff;
         const T* __restrict__ R = reinterpret_cast<const T*>(b.val[C.rslot]) + C.roff;
-        const T a = (C.flags & CF_RUNTIME_ALPHA) ? alpha * T(C.alpha) : T(C.alpha);
+        const T a = (C.flags & CF_RUNTIME_ALPHA) ? alpha * T(C.alpha)
+                  : (C.flags & CF_RUNTIME_BETA) ? beta * T(C.alpha) : T(C.alpha);
         const bool tL = C.flags & CF_TRANS_L, tR = C.flags & CF_TRANS_R;
         for (int p0 = 0; p0 < q; p0 += QC) {
             const int qc = min(QC, q - p0);
@@ -646,6 +647,19 @@ template <typename T> void launch_scale(T* x, int64_t n, T alpha, hipStream_t s)
 }
 
 template <typename T>
+__global__ void axpby_kernel(const T* a, const T* b, T alpha, T* out, int64_t n)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        out[e] = a[e] + alpha * b[e];
+}
+template <typename T> void launch_axpby(const T* a, const T* b, T alpha, T* out, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    axpby_kernel<T><<<grid_for(n), 256, 0, s>>>(a, b, alpha, out, n);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename T>
 __global__ void fill_kernel(T* x, int64_t n, T v)
 {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
@@ -820,6 +834,7 @@ void launch_zero_double(double* x, int n, hipStream_t s)
     template void launch_mv_phase<T>(const Phase&, const Bases&, T, T, hipStream_t);                \
     template void launch_scale<T>(T*, int64_t, T, hipStream_t);                                     \
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                      \
+    template void launch_axpby<T>(const T*, const T*, T, T*, int64_t, hipStream_t);                 \
     template void launch_gaussian<T>(T*, int64_t, uint64_t, uint64_t, hipStream_t);                 \
     template void launch_add_diag<T>(T*, const int64_t*, const int*, int, T, hipStream_t);          \
     template void launch_transpose_dense<T>(const T*, T*, const int64_t*, const int64_t*, const int*, \

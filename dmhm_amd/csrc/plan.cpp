@@ -280,18 +280,26 @@ ProductTerms enumerateProductTerms(const Geometry& g)
 }
 
 namespace {
-struct WFactor {            // exact low-rank term W = PU PV^T
+struct WFactor {            // exact low-rank term W = PU PV^T (times the run-time scalar in cflag)
     Operand PU, PV;
     int level, t, s;        // boxes of PU rows (t) and PV rows (s) at `level`
     int rankIndex;          // rank array entry of the low-rank operand
     uint8_t rankSlot;
     int cap;
+    uint8_t cflag = 0;      // CF_RUNTIME_ALPHA / CF_RUNTIME_BETA scaling of the term
 };
 } // namespace
 
-Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, int maxRank,
-                         int oversample)
+Plan* Planner::hhAdd(const Layout& LA, const Layout& LB, const Layout& LC, int maxRank, int oversample)
 {
+    return hhProduct(LA, LB, LC, maxRank, oversample, HH_ADD);
+}
+
+Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, int maxRank,
+                         int oversample, int mode)
+{
+    const bool update = (mode == HH_UPDATE);
+    const bool add = (mode == HH_ADD);
     const Geometry& g = g_;
     const int kS = maxRank + oversample;
     if (kS > 64) throw std::invalid_argument("maxRank + oversample must be <= 64");
@@ -299,7 +307,12 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
 
     // 1. Enumerate the product recursion and give every low-rank term its
     //    factor panels.
-    ProductTerms terms = enumerateProductTerms(g);
+    ProductTerms terms;
+    if (!add) terms = enumerateProductTerms(g);
+    else {
+        const size_t nb = g.blocks.size();
+        terms.pushdowns.resize(nb); terms.lrTerms.resize(nb); terms.nnTerms.resize(nb); terms.ddTerms.resize(nb);
+    }
     std::vector<WFactor> W(terms.w.size());
     struct Pending { int bA, bB; int wIndex; bool left; };
     std::vector<Pending> wPending;
@@ -327,10 +340,44 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         }
         wPending.push_back({tw.bA, tw.bB, (int)wi, tw.leftLowRank});
     }
+    const uint8_t abFlag = update ? CF_RUNTIME_ALPHA : 0;     // scaling of the A B terms in sketches
+    const uint8_t formFlag = (update || add) ? 0 : CF_RUNTIME_ALPHA;    // scaling applied when forming C.U
+    for (auto& w : W) w.cflag = abFlag;
     const auto& pushdowns = terms.pushdowns;
     const auto& lrTerms = terms.lrTerms;
     const auto& nnTerms = terms.nnTerms;
     const auto& ddTerms = terms.ddTerms;
+    // beta C_old for every admissible C block (update form); A and alpha B for
+    // the blockwise sum (HMat2d::UpdateWith).
+    std::vector<int> oldTerm(g.blocks.size(), -1), addTerm(g.blocks.size(), -1);
+    if (add)
+        for (int lb : g.lrBlocks) {
+            const Block& C = g.blocks[lb];
+            for (int which = 0; which < 2; ++which) {
+                const Layout& LX = which ? LB : LA;
+                WFactor w;
+                w.level = C.level; w.t = C.t; w.s = C.s;
+                w.cap = LX.cap[C.lrIndex];
+                const uint8_t slot = which ? SLOT_B : SLOT_A;
+                w.PU = {slot, LX.uOff[C.lrIndex], g.boxes[C.level][C.t].size};
+                w.PV = {slot, LX.vOff[C.lrIndex], g.boxes[C.level][C.s].size};
+                w.rankIndex = C.lrIndex; w.rankSlot = slot; w.cflag = which ? CF_RUNTIME_ALPHA : 0;
+                if (which == 0) addTerm[lb] = (int)W.size();
+                W.push_back(w);
+            }
+        }
+    if (update)
+        for (int lb : g.lrBlocks) {
+            const Block& C = g.blocks[lb];
+            WFactor w;
+            w.level = C.level; w.t = C.t; w.s = C.s;
+            w.cap = LC.cap[C.lrIndex];
+            w.PU = {SLOT_X, LC.uOff[C.lrIndex], g.boxes[C.level][C.t].size};
+            w.PV = {SLOT_X, LC.vOff[C.lrIndex], g.boxes[C.level][C.s].size};
+            w.rankIndex = C.lrIndex; w.rankSlot = SLOT_X; w.cflag = CF_RUNTIME_BETA;
+            oldTerm[lb] = (int)W.size();
+            W.push_back(w);
+        }
     // Ancestor pushdowns of every C leaf (ImportLowRank / UpdateWithLowRank
     // distribute a low-rank W over the subtree of a NODE, HMat2d.cpp:811-959).
     std::vector<std::vector<int>> leafW(g.blocks.size());
@@ -345,6 +392,8 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     for (size_t b = 0; b < g.blocks.size(); ++b) {
         auto& dst = leafW[b];
         dst.insert(dst.end(), lrTerms[b].begin(), lrTerms[b].end());
+        if (oldTerm[b] >= 0) dst.push_back(oldTerm[b]);
+        if (addTerm[b] >= 0) { dst.push_back(addTerm[b]); dst.push_back(addTerm[b] + 1); }
     }
 
     auto newPhase = [&](PhaseKind k) -> Phase& {
@@ -454,11 +503,11 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     // Dense accumulation of a leaf-level C block: out (|t| x |s|) =
     // sum D_A D_B + sum PU PV^T over its dense*dense and low-rank terms.
     auto denseAccumulate = [&](PhaseBuilder& acc, int b, const Operand& out,
-                               const std::vector<std::pair<int, int>>& dd, uint8_t cflags) {
+                               const std::vector<std::pair<int, int>>& dd, uint8_t cflags, uint8_t tflags) {
         const Block& C = g.blocks[b];
         const Box& tb = g.boxes[C.level][C.t];
         const Box& sb = g.boxes[C.level][C.s];
-        PanelTiles pt = addPanelTiles(acc, out, C.level, C.t, sb.size, -1, 0, 0);
+        PanelTiles pt = addPanelTiles(acc, out, C.level, C.t, sb.size, -1, 0, tflags);
         for (const auto& tm : dd) {
             const Block& A = g.blocks[tm.first];
             const Block& B = g.blocks[tm.second];
@@ -480,7 +529,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 for (int c = 0; c < pt.kChunks; ++c)
                     acc.add(pt.tileOfAtom[(size_t)(a - tb.atomBegin) * pt.kChunks + c],
                             mkc(pu.slot, pu.off + ra, pu.ld, pv.slot, pv.off + 64 * c, pv.ld, w.cap, w.rankIndex,
-                                w.rankSlot, CF_TRANS_R | cflags, 1.0f));
+                                w.rankSlot, CF_TRANS_R | (w.cflag ? w.cflag : cflags), 1.0f));
             }
             plan->flops += 2.0 * tb.size * w.cap * sb.size;
         }
@@ -488,13 +537,14 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     {
         Phase& ad = newPhase(PH_TILE);
         PhaseBuilder acc(ad);
-        for (int bd : g.denseBlocks) {
+        for (int bd : add ? std::vector<int>() : g.denseBlocks) {
             const Block& C = g.blocks[bd];
+            // Dense C leaves: C = alpha (sum of terms) (+ beta C_old, read in place).
             denseAccumulate(acc, bd, {SLOT_C, LC.denseOff[C.denseIndex], g.boxes[C.level][C.t].size},
-                            ddTerms[bd], CF_RUNTIME_ALPHA);
+                            ddTerms[bd], CF_RUNTIME_ALPHA, update ? TF_RUNTIME_BETA : 0);
         }
         for (const auto& s : sk)
-            if (s.leaf) denseAccumulate(acc, s.b, {SLOT_WS, s.D, s.m}, nnTerms[s.b], 0);
+            if (s.leaf) denseAccumulate(acc, s.b, {SLOT_WS, s.D, s.m}, nnTerms[s.b], abFlag, 0);
         acc.finalize();
     }
 
@@ -529,14 +579,14 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                     const int ra = g.atomStart[a] - tb.off;
                     acc.add(pt.tileOfAtom[a - tb.atomBegin],
                             mkc(pu.slot, pu.off + ra, pu.ld, SLOT_WS, P, w.cap, w.cap, w.rankIndex,
-                                w.rankSlot, 0, 1.0f));
+                                w.rankSlot, w.cflag, 1.0f));
                 }
                 plan->flops += 2.0 * tb.size * w.cap * kS;
             }
             for (const auto& tm : nnTerms[s.b]) {
                 const Block& B = g.blocks[tm.second];
                 const Operand in{SLOT_WS, t1Of[(uint64_t)tm.second], g.boxes[B.level][B.t].size};
-                subh(proj, acc, SLOT_A, LA, tm.first, false, in, B.level, B.t, pt, kS, -1, 0, 1.0f, 0,
+                subh(proj, acc, SLOT_A, LA, tm.first, false, in, B.level, B.t, pt, kS, -1, 0, 1.0f, abFlag,
                      &plan->flops);
             }
         }
@@ -592,7 +642,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                     const int ra = g.atomStart[a] - sb.off;
                     acc4.add(zt.tileOfAtom[a - sb.atomBegin],
                              mkc(pv.slot, pv.off + ra, pv.ld, SLOT_WS, P, w.cap, w.cap, w.rankIndex,
-                                 w.rankSlot, 0, 1.0f));
+                                 w.rankSlot, w.cflag, 1.0f));
                 }
                 plan->flops += 2.0 * sb.size * w.cap * kS;
             }
@@ -604,7 +654,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 subh(proj3, acc3, SLOT_A, LA, tm.first, true, Q, C.level, C.t, pt, kS, -1, 0, 1.0f, 0,
                      &plan->flops);
                 subh(proj4, acc4, SLOT_B, LB, tm.second, true, {SLOT_WS, T2, ub.size}, A.level, A.s, zt, kS,
-                     -1, 0, 1.0f, 0, &plan->flops);
+                     -1, 0, 1.0f, abFlag, &plan->flops);
             }
         }
         proj3.finalize(); acc3.finalize(); proj4.finalize(); acc4.finalize();
@@ -642,7 +692,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 for (int c = 0; c < ut.kChunks; ++c)
                     acc.add(ut.tileOfAtom[(size_t)(a - tb.atomBegin) * ut.kChunks + c],
                             mkc(SLOT_WS, s.Y + (g.atomStart[a] - tb.off), s.m, SLOT_WS,
-                                s.G1 + (int64_t)c * 64 * kS, kS, kS, -1, 0, CF_RUNTIME_ALPHA, 1.0f));
+                                s.G1 + (int64_t)c * 64 * kS, kS, kS, -1, 0, formFlag, 1.0f));
             for (int a = sb.atomBegin; a < sb.atomEnd; ++a)
                 for (int c = 0; c < vt.kChunks; ++c)
                     acc.add(vt.tileOfAtom[(size_t)(a - sb.atomBegin) * vt.kChunks + c],

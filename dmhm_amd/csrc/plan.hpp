@@ -27,7 +27,7 @@ namespace dmhm {
 enum Slot : uint8_t { SLOT_A = 0, SLOT_B = 1, SLOT_C = 2, SLOT_WS = 3, SLOT_X = 4, SLOT_Y = 5,
                       NUM_SLOTS = 8 };
 
-enum ContribFlags : uint8_t { CF_TRANS_L = 1, CF_TRANS_R = 2, CF_RUNTIME_ALPHA = 4 };
+enum ContribFlags : uint8_t { CF_TRANS_L = 1, CF_TRANS_R = 2, CF_RUNTIME_ALPHA = 4, CF_RUNTIME_BETA = 8 };
 enum TileFlags : uint8_t { TF_ACCUMULATE = 1, TF_RUNTIME_BETA = 2 };
 
 // Plain-old-data records shared with the kernels.
@@ -186,8 +186,15 @@ public:
     // block (sketch width kS = maxRank + oversample), A/B/C layouts given.
     // tolMode 0: keep min(maxRank, kS) singular values; 1: keep those above
     // relTol * sigma_0 (at most maxRank).
+    // update = true: C := alpha A B + beta C_old with C_old (layout LC) bound in
+    // SLOT_X; every term of A B is scaled by the run-time alpha and C_old by the
+    // run-time beta inside the sketch.
+    enum HHMode { HH_PRODUCT = 0, HH_UPDATE = 1, HH_ADD = 2 };
     Plan* hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, int maxRank,
-                    int oversample);
+                    int oversample, int mode = HH_PRODUCT);
+    // C := A + alpha B blockwise (HMat2d::UpdateWith): dense leaves summed,
+    // admissible blocks recompressed from a sketch of the exact sum.
+    Plan* hhAdd(const Layout& LA, const Layout& LB, const Layout& LC, int maxRank, int oversample);
 
     int64_t wsAlloc(int64_t elems) {
         const int64_t o = ws_;

@@ -106,6 +106,30 @@ class _HMatBase:
                                                 precision, ctypes.byref(out)))
         return cls(out)
 
+    @classmethod
+    def from_lowrank(cls, sizes, num_levels, max_rank, strongly_admissible, U, V, precision=DMHM_F64):
+        """HMat2d(const LowRank<Scalar>&, numLevels, maxRank, strong, xSize, ySize)
+        (src/HMat2d/HMat2d.cpp:124-142, ImportLowRank :811-904): F = U V^T with
+        rows in the matrix's hierarchical ordering; each admissible block takes
+        the matching row slices of U and V exactly (rank r, not truncated),
+        each dense leaf their product."""
+        _lib.init()
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        if len(sizes) != cls.dim:
+            raise ValueError("expected %d grid sizes" % cls.dim)
+        n = int(np.prod(sizes))
+        U = np.asfortranarray(U, dtype=np.float64)
+        V = np.asfortranarray(V, dtype=np.float64)
+        if U.ndim != 2 or V.ndim != 2 or U.shape[0] != n or V.shape[0] != n or U.shape[1] != V.shape[1]:
+            raise ValueError("U and V must be n x r with n = %d" % n)
+        dp = ctypes.POINTER(ctypes.c_double)
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hmat_create_from_lowrank(
+            cls.dim, sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), num_levels, max_rank,
+            1 if strongly_admissible else 0, precision, U.shape[1], U.ctypes.data_as(dp), V.ctypes.data_as(dp),
+            ctypes.byref(out)))
+        return cls(out)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value and _lib._lib is not None:
@@ -196,10 +220,10 @@ class _HMatBase:
 
     def Multiply(self, alpha, B, beta=None, C=None, **hh_options):
         """Multiply(alpha, x|X, [beta, y|Y]) -> alpha H X (+ beta Y), or
-        Multiply(alpha, HMat B, **hh_options) -> C := alpha H B
-        (MultiplyHMat-incl.hpp:12-187)."""
+        Multiply(alpha, HMat B, [beta, HMat C], **hh_options) -> C := alpha H B
+        (+ beta C) (MultiplyHMat-incl.hpp:12-380)."""
         if isinstance(B, _HMatBase):
-            return self.MultiplyHMat(alpha, B, **hh_options)
+            return self.MultiplyHMat(alpha, B, beta, C, **hh_options)
         return self._mult("N", alpha, B, 0.0 if beta is None else beta, C)
 
     def TransposeMultiply(self, alpha, B, beta=None, C=None):
@@ -207,11 +231,24 @@ class _HMatBase:
 
     AdjointMultiply = TransposeMultiply
 
-    def MultiplyHMat(self, alpha, B, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1):
-        out = ctypes.c_void_p()
+    def MultiplyHMat(self, alpha, B, beta=None, C=None, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1):
+        """C := alpha A B (new matrix), or C := alpha A B + beta C in place when
+        ``C`` is given (MultiplyHMat-incl.hpp:12-380)."""
         o = _lib.HHOpts(oversample, tol_mode, rel_tol, seed)
+        if C is not None:
+            check(_lib.load().dmhm_hmat_multiply_hmat_update(alpha, self._h, B._h, 0.0 if beta is None else beta,
+                                                             C._h, ctypes.byref(o)))
+            return C
+        out = ctypes.c_void_p()
         check(_lib.load().dmhm_hmat_multiply_hmat(alpha, self._h, B._h, ctypes.byref(out), ctypes.byref(o)))
         return type(self)(out)
+
+    def UpdateWith(self, alpha, B, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1):
+        """A := A + alpha B with recompression of admissible blocks
+        (src/HMat2d/HMat2d.cpp:746-784)."""
+        o = _lib.HHOpts(oversample, tol_mode, rel_tol, seed)
+        check(_lib.load().dmhm_hmat_update_with(self._h, alpha, B._h, ctypes.byref(o)))
+        return self
 
     # -- whole-matrix routines (HMat2d.cpp:331-744) --------------------------
     def Copy(self):
@@ -291,6 +328,50 @@ class HMat2d(_HMatBase):
 
 class HMat3d(_HMatBase):
     dim = 3
+
+
+def _fortran(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def RoundedUpdate(max_rank, alpha, A, beta, B):
+    """hmat_tools::RoundedUpdate (src/hmat_tools/RoundedUpdate.cpp:14-162):
+    returns the low-rank factors (U, V) of alpha A + beta B truncated to
+    max_rank, where A = (AU, AV) and B = (BU, BV) are low-rank pairs.
+    Exact concatenation [beta B, alpha A] when the summed rank fits."""
+    _lib.init()
+    AU, AV = map(_fortran, A)
+    BU, BV = map(_fortran, B)
+    m, n = AU.shape[0], AV.shape[0]
+    if BU.shape[0] != m or BV.shape[0] != n or AU.shape[1] != AV.shape[1] or BU.shape[1] != BV.shape[1]:
+        raise ValueError("nonconformal low-rank operands")
+    rA, rB = AU.shape[1], BU.shape[1]
+    r = rA + rB
+    outU = np.zeros((m, max(r, 1)), order="F")
+    outV = np.zeros((n, max(r, 1)), order="F")
+    keep = ctypes.c_int()
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(_lib.load().dmhm_rounded_update(max_rank, alpha, m, n, rA, AU.ctypes.data_as(dp), AV.ctypes.data_as(dp),
+                                          beta, rB, BU.ctypes.data_as(dp), BV.ctypes.data_as(dp),
+                                          outU.ctypes.data_as(dp), outV.ctypes.data_as(dp), ctypes.byref(keep)))
+    return outU[:, :keep.value].copy(order="F"), outV[:, :keep.value].copy(order="F")
+
+
+def Compress(max_rank, D):
+    """hmat_tools::Compress of a dense block (src/hmat_tools/Compress.cpp:14-53):
+    returns (U, V) with D ~= U V^T truncated to max_rank by an SVD
+    (V carries the singular values)."""
+    _lib.init()
+    D = _fortran(D)
+    m, n = D.shape
+    k = min(m, n)
+    outU = np.zeros((m, max(k, 1)), order="F")
+    outV = np.zeros((n, max(k, 1)), order="F")
+    keep = ctypes.c_int()
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(_lib.load().dmhm_compress_dense(max_rank, m, n, D.ctypes.data_as(dp), outU.ctypes.data_as(dp),
+                                          outV.ctypes.data_as(dp), ctypes.byref(keep)))
+    return outU[:, :keep.value].copy(order="F"), outV[:, :keep.value].copy(order="F")
 
 
 def EstimateTwoNorm(A, theta=1.5, confidence=6.0, seed=1):

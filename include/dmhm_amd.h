@@ -141,6 +141,43 @@ typedef struct {
 int dmhm_hmat_multiply_hmat(double alpha, dmhm_hmat A, dmhm_hmat B, dmhm_hmat* C,
                             const dmhm_hh_opts* opts);
 
+/* C := alpha A B + beta C, HMat2d::Multiply(alpha, B, beta, C)
+ * (src/HMat2d/MultiplyHMat-incl.hpp:189-380). C keeps its geometry; every
+ * admissible block of C is recompressed from a sketch of the exact
+ * alpha A B + beta C restricted to it. */
+int dmhm_hmat_multiply_hmat_update(double alpha, dmhm_hmat A, dmhm_hmat B, double beta, dmhm_hmat C,
+                                   const dmhm_hh_opts* opts);
+
+/* A :~= alpha B + A, HMat2d::UpdateWith (src/HMat2d/HMat2d.cpp:746-784):
+ * dense leaves summed exactly, admissible blocks recompressed to maxRank
+ * (RoundedUpdate semantics, src/hmat_tools/RoundedUpdate.cpp:14-162). */
+int dmhm_hmat_update_with(dmhm_hmat A, double alpha, dmhm_hmat B, const dmhm_hh_opts* opts);
+
+/* HMat2d(const LowRank<Scalar>& F, numLevels, maxRank, strong, xSize, ySize)
+ * (src/HMat2d/HMat2d.cpp:124-142 + ImportLowRank :811-904): admissible blocks
+ * take the row slices of U (n x r) and V (n x r) exactly (rank r, not
+ * truncated), dense leaves U_t V_s^T. Host arrays, column-major, ld = n. */
+int dmhm_hmat_create_from_lowrank(int dim, const int* sizes, int numLevels, int maxRank, int strong,
+                                  int precision, int r, const double* U, const double* V, dmhm_hmat* out);
+
+/* hmat_tools::RoundedUpdate (src/hmat_tools/RoundedUpdate.cpp:14-162):
+ * B :~= alpha A + beta B for m x n low-rank A = AU AV^T (rank rA) and
+ * B = BU BV^T (rank rB). If rA + rB <= min(maxRank, m, n) the result is the
+ * exact concatenation [beta BU, alpha AU], [BV, AV]; otherwise QR of both
+ * stacked factors, SVD of R1 R2^T on the GPU, truncation to
+ * min(maxRank, m, n) (rA + rB <= 64). outU (m x .), outV (n x .) receive the
+ * result (capacity rA + rB columns), *outRank its rank. Host arrays. */
+int dmhm_rounded_update(int maxRank, double alpha, int m, int n, int rA, const double* AU,
+                        const double* AV, double beta, int rB, const double* BU, const double* BV,
+                        double* outU, double* outV, int* outRank);
+
+/* hmat_tools::Compress (src/hmat_tools/Compress.cpp:14-53): best rank
+ * min(maxRank, m, n) approximation D ~= U V^T of an m x n dense matrix
+ * (Householder QR + Jacobi SVD on the GPU; min(m, n) <= 64). Host arrays;
+ * outU (m x min(m,n)), outV (n x min(m,n)) capacity. */
+int dmhm_compress_dense(int maxRank, int m, int n, const double* D, double* outU, double* outV,
+                        int* outRank);
+
 /* HMat2d::CopyFrom / TransposeFrom / AdjointFrom / Scale / SetToIdentity /
  * AddConstantToDiagonal (src/HMat2d/HMat2d.cpp:331-744). */
 int dmhm_hmat_copy(dmhm_hmat src, dmhm_hmat* dst);

@@ -257,6 +257,11 @@ class HMat:
     def matmul(self, other, alpha=1.0):
         return HMat(lib().orc_hmat_multiply(alpha, self.ptr, other.ptr))
 
+    def multiply_update(self, alpha, A, B, beta):
+        """self := alpha A B + beta self (in place)."""
+        lib().orc_hmat_multiply_update(alpha, A.ptr, B.ptr, beta, self.ptr)
+        return self
+
     def copy(self):
         return HMat(lib().orc_hmat_copy(self.ptr))
 

@@ -310,3 +310,111 @@ def test_bad_arguments_raise():
     H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 4, True, *_args(op))
     with pytest.raises(dmhm_amd.DmhmError):
         H.EstimateTwoNorm(theta=1.0)
+
+
+# --- update forms, low-rank import, RoundedUpdate / Compress -----------------
+
+@pytest.mark.parametrize("name,n,L,rank,strong", [("c1-64", 64, 4, 8, True), ("16-weak-r16", 16, 3, 16, False)])
+def test_hh_update_form_matches_oracle(name, n, L, rank, strong):
+    """C := -A A + 0.5 C with C = A (MultiplyHMat-incl.hpp:189-380); the
+    operands are exact at these ranks, so both implementations agree."""
+    op = W.small_operator(n, L)
+    H = dmhm_amd.HMat2d.from_csr(op["sizes"], L, rank, strong, *_args(op))
+    C = H.Copy()
+    H.Multiply(-1.0, H, 0.5, C)
+    R = orc.HMat.from_csr(2, op["sizes"], L, rank, strong, *_args(op))
+    Cr = R.copy()
+    orc.rng_initialize()
+    Cr.multiply_update(-1.0, R, R, 0.5)
+    assert _relmax(C.to_dense(), Cr.to_dense()) < 1e-12
+
+
+@pytest.mark.parametrize("dim,n,L,rank,strong", [(2, 16, 3, 16, True), (3, 8, 3, 8, True)])
+def test_hh_update_form_exact_on_random_operands(dim, n, L, rank, strong):
+    op = W.small_operator(n, L, dim=dim)
+    cls = _cls(dim)
+    A, B, C = (cls.from_csr(op["sizes"], L, rank, strong, *_args(op)) for _ in range(3))
+    A.SetToRandom(seed=1)
+    B.SetToRandom(seed=2)
+    C.SetToRandom(seed=3)
+    ref = 0.75 * A.to_dense() @ B.to_dense() - 0.25 * C.to_dense()
+    A.Multiply(0.75, B, -0.25, C)
+    assert np.linalg.norm(C.to_dense() - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_update_with_exact_sum():
+    """A := A + alpha B at full block rank: exact."""
+    op = W.small_operator(16, 3)
+    A = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 16, True, *_args(op))
+    B = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 16, True, *_args(op))
+    A.SetToRandom(seed=5)
+    B.SetToRandom(seed=6)
+    ref = A.to_dense() + 0.3 * B.to_dense()
+    A.UpdateWith(0.3, B)
+    assert np.linalg.norm(A.to_dense() - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_update_with_truncation_is_optimal():
+    """Two rank-4 operands summed and rounded to rank 4: the sketch (4 + 4
+    oversampling columns) spans the whole rank-8 range of every block, so the
+    result is the best rank-4 truncation (RoundedUpdate's QR + SVD result):
+    the error equals the blockwise Eckart-Young floor."""
+    op = W.small_operator(16, 3)
+    A = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 4, False, *_args(op))
+    B = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 4, False, *_args(op))
+    A.SetToRandom(seed=7)
+    B.SetToRandom(seed=8)
+    P = A.to_dense() - 0.6 * B.to_dense()
+    A.UpdateWith(-0.6, B)
+    leaves = dmhm_amd.geometry_leaves(op["sizes"], 3, False)
+    floor = _blockwise_floor(P, leaves, 4)
+    err = np.linalg.norm(A.to_dense() - P)
+    assert floor > 1e-3 * np.linalg.norm(P)
+    assert abs(err - floor) <= 1e-9 * np.linalg.norm(P)
+
+
+@pytest.mark.parametrize("r,max_rank", [(3, 4), (6, 4)])
+def test_import_lowrank_is_exact(r, max_rank):
+    """HMat2d(LowRank F, ...) (HMat2d.cpp:124-142, ImportLowRank): admissible
+    blocks keep the exact factors (rank r even above maxRank), dense leaves
+    hold U_t V_s^T, so the H-matrix equals U V^T."""
+    rng = np.random.default_rng(11)
+    n = 256
+    U = rng.standard_normal((n, r))
+    V = rng.standard_normal((n, r))
+    H = dmhm_amd.HMat2d.from_lowrank([16, 16], 3, max_rank, True, U, V)
+    assert _relmax(H.to_dense(), U @ V.T) < 1e-13
+    assert set(np.unique(H.ranks())) == {r}
+    X = rng.standard_normal((n, 3))
+    assert _relmax(H.Multiply(1.0, X), U @ (V.T @ X)) < 1e-12
+
+
+@pytest.mark.parametrize("m,n,rA,rB,max_rank", [(40, 30, 5, 7, 6), (5, 40, 4, 8, 3), (64, 64, 30, 34, 16)])
+def test_rounded_update_matches_oracle(m, n, rA, rB, max_rank):
+    rng = np.random.default_rng(m + n + rA)
+    AU, AV = rng.standard_normal((m, rA)), rng.standard_normal((n, rA))
+    BU, BV = rng.standard_normal((m, rB)), rng.standard_normal((n, rB))
+    U, V = dmhm_amd.RoundedUpdate(max_rank, 0.7, (AU, AV), -1.3, (BU, BV))
+    Ur, Vr = orc.rounded_update(max_rank, 0.7, AU, AV, -1.3, BU, BV)
+    assert U.shape[1] == Ur.shape[1] == min(max_rank, m, n)
+    assert _relmax(U @ V.T, Ur @ Vr.T) < 1e-11
+    assert np.abs(U.T @ U - np.eye(U.shape[1])).max() < 1e-12
+
+
+def test_rounded_update_early_exit_concatenates():
+    rng = np.random.default_rng(3)
+    AU, AV = rng.standard_normal((20, 3)), rng.standard_normal((25, 3))
+    BU, BV = rng.standard_normal((20, 2)), rng.standard_normal((25, 2))
+    U, V = dmhm_amd.RoundedUpdate(8, 2.0, (AU, AV), 0.5, (BU, BV))
+    Ur, Vr = orc.rounded_update(8, 2.0, AU, AV, 0.5, BU, BV)
+    assert np.array_equal(U, Ur) and np.array_equal(V, Vr)
+
+
+@pytest.mark.parametrize("m,n,max_rank", [(50, 20, 8), (10, 30, 6), (20, 90, 5), (64, 64, 64)])
+def test_compress_matches_oracle(m, n, max_rank):
+    rng = np.random.default_rng(m * n)
+    D = rng.standard_normal((m, n)) * np.logspace(0, -6, n)
+    U, V = dmhm_amd.Compress(max_rank, D)
+    Ur, Vr = orc.compress_dense(max_rank, D)
+    assert U.shape[1] == Ur.shape[1] == min(max_rank, m, n)
+    assert _relmax(U @ V.T, Ur @ Vr.T) < 1e-11
