@@ -1,0 +1,266 @@
+// Sequential-caller test of the C++ facade (include/dmhm_amd/hmat.hpp): the
+// calls follow the reference's own drivers (tests/HMat2d/Invert.cpp:128-330,
+// tests/HMat2d/MultiplyHMat.cpp, tests/hmat_tools/Compress.cpp) against the
+// dmhm:: names. Modes:
+//   map    print the 2D hierarchical map of an 8 x 4 grid, numLevels 3 (no GPU)
+//   nogpu  Initialize() must throw std::runtime_error (no GPU present)
+//   gpu    construct / multiply / H*H / UpdateWith / Schulz / tools on GPU 0,
+//          checking each result; exit status 0 when every check passes
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+
+#include "dmhm_amd/hmat.hpp"
+
+using namespace dmhm;
+
+static int failures = 0;
+static void expect(bool ok, const char* what, double value)
+{
+    std::printf("%-44s %-4s %.3e\n", what, ok ? "ok" : "FAIL", value);
+    if (!ok) ++failures;
+}
+
+// 5-point -Laplace(u) + u on an n x n grid (h = 1/(n+1)), rows and columns in
+// hierarchical ordering, like the reference driver's FormRow loop.
+template <typename Scalar>
+static Sparse<Scalar> laplacian(int n, int numLevels)
+{
+    Vector<int> map;
+    HMat2d<Scalar>::BuildNaturalToHierarchicalMap(map, n, n, numLevels);
+    const int m = n * n;
+    Vector<int> inverseMap(m);
+    for (int i = 0; i < m; ++i) inverseMap[map[i]] = i;
+    const double h = 1.0 / (n + 1), s = 1.0 / (h * h);
+    Sparse<Scalar> S;
+    S.height = S.width = m;
+    S.symmetric = false;
+    for (int i = 0; i < m; ++i) {
+        S.rowOffsets.PushBack(S.nonzeros.Size());
+        const int nat = inverseMap[i], x = nat % n, y = nat / n;
+        S.nonzeros.PushBack(Scalar(4 * s + 1));
+        S.columnIndices.PushBack(i);
+        const int nb[4][2] = {{x - 1, y}, {x + 1, y}, {x, y - 1}, {x, y + 1}};
+        for (auto& q : nb)
+            if (q[0] >= 0 && q[0] < n && q[1] >= 0 && q[1] < n) {
+                S.nonzeros.PushBack(Scalar(-s));
+                S.columnIndices.PushBack(map[q[0] + n * q[1]]);
+            }
+    }
+    S.rowOffsets.PushBack(S.nonzeros.Size());
+    return S;
+}
+
+template <typename Scalar>
+static void spmv(const Sparse<Scalar>& S, const Vector<Scalar>& x, Vector<double>& y)
+{
+    y.Resize(S.height);
+    for (int i = 0; i < S.height; ++i) {
+        double sum = 0;
+        for (int e = S.rowOffsets[i]; e < S.rowOffsets[i + 1]; ++e)
+            sum += (double)S.nonzeros[e] * (double)x[S.columnIndices[e]];
+        y[i] = sum;
+    }
+}
+
+template <typename V1, typename V2>
+static double rel_diff(const V1& a, const V2& b, int n)
+{
+    double num = 0, den = 0;
+    for (int i = 0; i < n; ++i) {
+        const double d = (double)a[i] - (double)b[i];
+        num += d * d;
+        den += (double)b[i] * (double)b[i];
+    }
+    return std::sqrt(num / (den > 0 ? den : 1));
+}
+
+static int run_gpu()
+{
+    Initialize(0);
+    const int n = 32, L = 3, m = n * n;
+    const Sparse<double> S = laplacian<double>(n, L);
+    HMat2d<double> H(S, L, 8, true, n, n);
+    expect(H.Height() == m && H.NumLevels() == L && H.MaxRank() == 8 && H.StronglyAdmissible(),
+           "HMat2d(Sparse) shape", 0);
+
+    Vector<double> x(m), y, z;
+    for (int i = 0; i < m; ++i) x[i] = 1.0 + 0.001 * (i % 17);
+    H.Multiply(1.0, x, y);
+    Vector<double> ref;
+    spmv(S, x, ref);
+    double e = rel_diff(y, ref, m);
+    expect(e < 1e-13, "Multiply(alpha, x, y) == S x", e);
+    H.TransposeMultiply(1.0, x, z);
+    e = rel_diff(z, ref, m);   // the stencil is symmetric
+    expect(e < 1e-13, "TransposeMultiply == S^T x", e);
+
+    Dense<double> X(m, 3), Y;
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < m; ++i) X.Set(i, j, std::sin(0.01 * i + j));
+    H.Multiply(2.0, X, Y);
+    double worst = 0;
+    for (int j = 0; j < 3; ++j) {
+        Vector<double> xj(m), yj(m);
+        for (int i = 0; i < m; ++i) xj[i] = X.Get(i, j);
+        spmv(S, xj, ref);
+        for (int i = 0; i < m; ++i) yj[i] = Y.Get(i, j) / 2.0;
+        worst = std::fmax(worst, rel_diff(yj, ref, m));
+    }
+    expect(worst < 1e-13, "Multiply(alpha, Dense, Dense)", worst);
+
+    // C := A A is exact here (admissible blocks of the squared stencil have
+    // rank <= 2 x 8 and strong admissibility leaves them empty).
+    HMat2d<double> C;
+    H.Multiply(1.0, H, C);
+    Vector<double> hx, hhx, cx;
+    H.Multiply(1.0, x, hx);
+    H.Multiply(1.0, hx, hhx);
+    C.Multiply(1.0, x, cx);
+    e = rel_diff(cx, hhx, m);
+    expect(e < 1e-12, "Multiply(alpha, HMat B, HMat C)", e);
+
+    // C := -A A + 2 C  (C = A A before) -> C = A A.
+    H.Multiply(-1.0, H, 2.0, C);
+    C.Multiply(1.0, x, cx);
+    e = rel_diff(cx, hhx, m);
+    expect(e < 1e-12, "Multiply(alpha, B, beta, C)", e);
+
+    HMat2d<double> D;
+    D.CopyFrom(H);
+    D.UpdateWith(-0.5, H);
+    D.Multiply(1.0, x, z);
+    for (int i = 0; i < m; ++i) ref[i] = 0.5 * y[i];
+    e = rel_diff(z, ref, m);
+    expect(e < 1e-13, "CopyFrom + UpdateWith", e);
+
+    HMat2d<double> T;
+    T.TransposeFrom(H);
+    T.Scale(3.0);
+    T.AddConstantToDiagonal(1.0);
+    T.Multiply(1.0, x, z);
+    for (int i = 0; i < m; ++i) ref[i] = 3.0 * y[i] + x[i];
+    e = rel_diff(z, ref, m);
+    expect(e < 1e-13, "TransposeFrom + Scale + AddConstantToDiagonal", e);
+
+    const double est = hmat_tools::EstimateTwoNorm(H, 1.5, 6.0);
+    const double hmax = 8.0 * (n + 1) * (n + 1) + 1.0;   // Gershgorin bound of the stencil
+    expect(est > 0.5 * hmax && est < 1.5 * hmax, "hmat_tools::EstimateTwoNorm", est);
+
+    HMat2d<double> X64;
+    X64.CopyFrom(H);
+    X64.SchulzInvert(0, 1.5, 6.0, 1e-6);
+    const dmhm_schulz_report& rep = X64.LastSchulzReport();
+    X64.Multiply(1.0, y, z);      // X (H x) ~= x
+    e = rel_diff(z, x, m);
+    expect(e < 1e-3, "SchulzInvert: X (H x) ~= x", e);
+    expect(rep.iterations > 0, "SchulzInvert report", rep.residualEstimate);
+
+    // LowRank constructor: H = U V^T exactly (tests/HMat2d/LowRankConstructor.cpp).
+    LowRank<double> F;
+    F.U.Resize(m, 2);
+    F.V.Resize(m, 2);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < 2; ++j) {
+            F.U.Set(i, j, std::cos(0.003 * i * (j + 1)));
+            F.V.Set(i, j, std::sin(0.002 * i + j));
+        }
+    HMat2d<double> G(F, L, 4, true, n, n);
+    G.Multiply(1.0, x, z);
+    for (int i = 0; i < m; ++i) {
+        double s = 0;
+        for (int j = 0; j < 2; ++j) {
+            double vx = 0;
+            for (int k = 0; k < m; ++k) vx += F.V.Get(k, j) * x[k];
+            s += F.U.Get(i, j) * vx;
+        }
+        ref[i] = s;
+    }
+    e = rel_diff(z, ref, m);
+    expect(e < 1e-12, "HMat2d(LowRank) == U V^T", e);
+
+    // hmat_tools::Compress of a rank-3 matrix and RoundedUpdate of two rank-2 terms.
+    Dense<double> M(40, 30);
+    for (int j = 0; j < 30; ++j)
+        for (int i = 0; i < 40; ++i) M.Set(i, j, (i + 1.0) * (j + 2.0) + std::sin(i) * std::cos(j) + 1.0);
+    LowRank<double> Fm;
+    hmat_tools::Compress(3, M, Fm);
+    worst = 0;
+    for (int j = 0; j < 30; ++j)
+        for (int i = 0; i < 40; ++i) {
+            double s = 0;
+            for (int r = 0; r < Fm.Rank(); ++r) s += Fm.U.Get(i, r) * Fm.V.Get(j, r);
+            worst = std::fmax(worst, std::fabs(s - M.Get(i, j)) / 1e3);
+        }
+    expect(Fm.Rank() == 3 && worst < 1e-12, "hmat_tools::Compress (rank 3 input)", worst);
+    LowRank<double> A2, B2;
+    A2.U.Resize(20, 2); A2.V.Resize(25, 2); B2.U.Resize(20, 2); B2.V.Resize(25, 2);
+    for (int i = 0; i < 20; ++i)
+        for (int j = 0; j < 2; ++j) { A2.U.Set(i, j, i + j); B2.U.Set(i, j, i - 2.0 * j); }
+    for (int i = 0; i < 25; ++i)
+        for (int j = 0; j < 2; ++j) { A2.V.Set(i, j, i - j); B2.V.Set(i, j, 1.0 + i * j); }
+    LowRank<double> B0 = B2;
+    hmat_tools::RoundedUpdate(8, 2.0, A2, 0.5, B2);   // rank 4 <= 8: exact concatenation
+    worst = 0;
+    for (int j = 0; j < 25; ++j)
+        for (int i = 0; i < 20; ++i) {
+            double want = 0, got = 0;
+            for (int r = 0; r < 2; ++r)
+                want += 2.0 * A2.U.Get(i, r) * A2.V.Get(j, r) + 0.5 * B0.U.Get(i, r) * B0.V.Get(j, r);
+            for (int r = 0; r < B2.Rank(); ++r) got += B2.U.Get(i, r) * B2.V.Get(j, r);
+            worst = std::fmax(worst, std::fabs(got - want));
+        }
+    expect(B2.Rank() == 4 && worst < 1e-10, "hmat_tools::RoundedUpdate (concatenation)", worst);
+
+    // Single precision instantiation.
+    const Sparse<float> Sf = laplacian<float>(n, L);
+    HMat2d<float> Hf(Sf, L, 8, true, n, n);
+    Vector<float> xf(m), yf;
+    for (int i = 0; i < m; ++i) xf[i] = (float)x[i];
+    Hf.Multiply(1.0f, xf, yf);
+    spmv(Sf, xf, ref);
+    e = rel_diff(yf, ref, m);
+    expect(e < 1e-5, "HMat2d<float>::Multiply", e);
+
+    // Error mapping: nonconformal H*H raises std::logic_error.
+    bool threw = false;
+    try {
+        HMat2d<double> small(laplacian<double>(16, 3), 3, 8, true, 16, 16), out;
+        H.Multiply(1.0, small, out);
+    } catch (const std::logic_error&) {
+        threw = true;
+    }
+    expect(threw, "nonconformal product -> std::logic_error", 0);
+    Finalize();
+    std::printf("%s\n", failures ? "FAILED" : "PASSED");
+    return failures ? 1 : 0;
+}
+
+int main(int argc, char** argv)
+{
+    const char* mode = argc > 1 ? argv[1] : "gpu";
+    if (!std::strcmp(mode, "map")) {
+        Vector<int> map;
+        HMat2d<double>::BuildNaturalToHierarchicalMap(map, 8, 4, 3);
+        for (int i = 0; i < map.Size(); ++i) std::printf("%d%c", map[i], i + 1 < map.Size() ? ' ' : '\n');
+        return 0;
+    }
+    if (!std::strcmp(mode, "nogpu")) {
+        try {
+            Initialize(0);
+        } catch (const std::runtime_error& e) {
+            std::printf("runtime_error: %s\n", e.what());
+            return 0;
+        }
+        std::printf("no exception\n");
+        return 1;
+    }
+    try {
+        return run_gpu();
+    } catch (const std::exception& e) {
+        std::printf("exception: %s\n", e.what());
+        return 2;
+    }
+}
