@@ -147,7 +147,8 @@ def main():
         Y.SetToRandom(seed=2)
         for _ in range(a.reps):
             ms, fl = X.profile_hh(Y)
-        print(json.dumps({"mode": "hh", "ms": ms.tolist(), "flops": fl.tolist()}))
+        print(json.dumps({"mode": "hh", "ms": ms.tolist(), "flops": fl.tolist(),
+                          "phases": X.last_hh_phases()}))
         return
 
     # Warm-up: plan building, workspace, one iteration in each precision.
@@ -210,6 +211,10 @@ def main():
     for name, M in (("f64", X), ("f32", X32)):
         msh, flh = M.profile_hh(M)
         msh, flh = M.profile_hh(M)
+        phases = M.last_hh_phases()
+        hh[name + "_phases"] = [{"kind": ["tile", "qr", "svd", "rng"][k] if k < 4 else str(k), "ms": t,
+                                 "tflops": (f / (t * 1e-3) / 1e12) if (t > 0 and f > 0) else None}
+                                for k, t, f in phases]
         hh[name] = {"ms": msh[0], "tile_ms": msh[1], "qr_ms": msh[2], "svd_ms": msh[3],
                     "tile_tflops": flh[0] / (msh[1] * 1e-3) / 1e12,
                     "longest_tile_launch_ms": msh[5],

@@ -98,6 +98,7 @@ SIGNATURES = {
     "dmhm_rounded_update": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
     "dmhm_compress_dense": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip]),
+    "dmhm_profile_hh_phases": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _ip, _ip]),
     "dmhm_hmat_copy": (ctypes.c_int, [_h, _hp]),
     "dmhm_hmat_transpose": (ctypes.c_int, [_h, _hp]),
     "dmhm_hmat_scale": (ctypes.c_int, [_h, ctypes.c_double]),

@@ -951,6 +951,9 @@ static void profile_matvec(dmhm_hmat_s* h, int k, int reps, double* ms, double* 
     bytes[1] = es * (dEnt + uEnt + (double)n * k);
 }
 
+struct PhaseTiming { int kind; double ms, flops; };
+static std::vector<PhaseTiming> g_lastPhases;
+
 static void profile_hh(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, const dmhm_hh_opts& o, double* ms,
                        double* flops)
 {
@@ -982,8 +985,10 @@ static void profile_hh(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, const dmhm_
     for (int i = 0; i < 6; ++i) ms[i] = 0;
     flops[0] = flops[1] = 0;
     ms[0] = all.ms();
+    g_lastPhases.clear();
     for (size_t i = 0; i < p.phases.size(); ++i) {
         const double t = ev[i].ms();
+        g_lastPhases.push_back({(int)p.phases[i].kind, t, p.phases[i].flops});
         switch (p.phases[i].kind) {
         case PH_TILE:
             ms[1] += t;
@@ -1421,6 +1426,19 @@ int dmhm_profile_hh(double alpha, dmhm_hmat A, dmhm_hmat B, const dmhm_hh_opts* 
     });
 }
 
+int dmhm_profile_hh_phases(int cap, double* ms, double* flops, int* kinds, int* count)
+{
+    return guard([&] {
+        if (!count) throw Error(DMHM_EINVAL, "null pointer");
+        *count = (int)g_lastPhases.size();
+        for (int i = 0; i < cap && i < (int)g_lastPhases.size(); ++i) {
+            if (ms) ms[i] = g_lastPhases[i].ms;
+            if (flops) flops[i] = g_lastPhases[i].flops;
+            if (kinds) kinds[i] = g_lastPhases[i].kind;
+        }
+    });
+}
+
 int dmhm_hmat_matvec_bytes(dmhm_hmat h, int k, double* bytes, double* flops)
 {
     return guard([&] {
@@ -1447,7 +1465,15 @@ void Plan::upload()
                                hipMemcpyHostToDevice));
         }
         const void* src = nullptr; size_t bytes = 0;
-        if (ph.kind == PH_QR && !ph.qr.empty()) { src = ph.qr.data(); bytes = ph.qr.size() * sizeof(QrTask); }
+        if (ph.kind == PH_QR && !ph.qr.empty()) {
+            // Panels that fit one wave first (independent tasks; order is free).
+            auto small = [](const QrTask& t) { return t.m <= 64 && t.k <= 40; };
+            auto mid = std::stable_partition(ph.qr.begin(), ph.qr.end(), small);
+            ph.qrSmall = (int)(mid - ph.qr.begin());
+            ph.qrSmallK = 0;
+            for (auto it = ph.qr.begin(); it != mid; ++it) ph.qrSmallK = std::max(ph.qrSmallK, it->k);
+            src = ph.qr.data(); bytes = ph.qr.size() * sizeof(QrTask);
+        }
         if (ph.kind == PH_SVD && !ph.svd.empty()) { src = ph.svd.data(); bytes = ph.svd.size() * sizeof(SvdTask); }
         if (ph.kind == PH_RNG && !ph.rng.empty()) { src = ph.rng.data(); bytes = ph.rng.size() * sizeof(RngTask); }
         if (ph.kind == PH_MV_PROJ && !ph.mvProj.empty()) {

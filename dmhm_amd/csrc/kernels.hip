@@ -24,105 +24,168 @@ namespace dmhm {
 
 // ---------------------------------------------------------------------------
 // Tile GEMM engine.
-// Workgroup = 256 threads = 4 waves. With MR = 64 rows, thread
-// (i = tid & 63, g = tid >> 6) owns row i and columns j = g + 4*kk; with
-// MR = 16 (small projection tiles) thread (i = tid & 15, g = tid >> 4) owns
-// row i and columns j = g + 16*kk. L (m x q) and R (q x k) are staged through
-// LDS in chunks of QC along q: all global loads of a chunk are issued before
-// any LDS store (latency overlap); R is pre-scaled by alpha.
+// Workgroup = 256 threads; a tile is at most MR x NC (MR = 64 or 16 rows,
+// NC = 16, 32 or 64 columns). Thread (ti, tj) = (tid % RT, tid / RT) owns
+// the TM x TN micro-tile at rows TM*ti.., columns TN*tj.., accumulated in
+// registers (TM + TN LDS reads per TM*TN FMAs). L and R are staged through
+// LDS in chunks of QC along the inner dimension, double-buffered: the global
+// loads of the next chunk (possibly of the next contribution) are issued
+// before the current chunk is multiplied, so one barrier per chunk suffices
+// and load latency overlaps the arithmetic. Out-of-range entries are staged
+// as zeros, so every chunk multiplies a full QC slice. R is pre-scaled by
+// the contribution's alpha when it is stored to LDS.
 // ---------------------------------------------------------------------------
-template <typename T> struct TileCfg { static constexpr int QC = 64; };
-template <> struct TileCfg<double> { static constexpr int QC = 32; };
+template <typename T> struct TileCfg { static constexpr int QC = 32; };
+template <> struct TileCfg<double> { static constexpr int QC = 16; };
 
-template <typename T, int KPT, int ROLE, int MR>
+__device__ __forceinline__ int contrib_q(const Bases& b, const Contrib& C)
+{
+    int q = C.q;
+    if (C.qrank >= 0) q = min(q, b.rank[C.qslot][C.qrank]);
+    return q;
+}
+
+template <typename T, int MR, int NC, int QC, int LN, int RN>
+__device__ __forceinline__ void tile_load(const Bases& b, const Contrib& C, int p0, int q, int m, int k,
+                                          T alpha, T beta, T (&lr)[LN], T (&rr)[RN], T& a)
+{
+    const int tid = threadIdx.x;
+    const int qc = min(QC, q - p0);
+    const T* __restrict__ L = reinterpret_cast<const T*>(b.val[C.lslot]) + C.loff;
+    const T* __restrict__ R = reinterpret_cast<const T*>(b.val[C.rslot]) + C.roff;
+    a = (C.flags & CF_RUNTIME_ALPHA) ? alpha * T(C.alpha)
+      : (C.flags & CF_RUNTIME_BETA) ? beta * T(C.alpha) : T(C.alpha);
+    const bool tL = C.flags & CF_TRANS_L, tR = C.flags & CF_TRANS_R;
+#pragma unroll
+    for (int u = 0; u < LN; ++u) {
+        const int e = tid + 256 * u;
+        int ii, pp;
+        if (!tL) { ii = e % MR; pp = e / MR; } else { pp = e % QC; ii = e / QC; }
+        const bool ok = e < MR * QC && ii < m && pp < qc;
+        lr[u] = ok ? (!tL ? L[ii + (int64_t)(p0 + pp) * C.ldl] : L[(p0 + pp) + (int64_t)ii * C.ldl]) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < RN; ++u) {
+        const int e = tid + 256 * u;
+        int jj, pp;
+        if (!tR) { pp = e % QC; jj = e / QC; } else { jj = e % NC; pp = e / NC; }
+        const bool ok = e < QC * NC && jj < k && pp < qc;
+        rr[u] = ok ? (!tR ? R[(p0 + pp) + (int64_t)jj * C.ldr] : R[jj + (int64_t)(p0 + pp) * C.ldr]) : T(0);
+    }
+}
+
+template <typename T, int MR, int NC, int QC, int LN, int RN, int LP, int RP>
+__device__ __forceinline__ void tile_store(T* __restrict__ Lb, T* __restrict__ Rb, const T (&lr)[LN],
+                                           const T (&rr)[RN], T a, uint8_t flags)
+{
+    const int tid = threadIdx.x;
+    const bool tL = flags & CF_TRANS_L, tR = flags & CF_TRANS_R;
+#pragma unroll
+    for (int u = 0; u < LN; ++u) {
+        const int e = tid + 256 * u;
+        if (e < MR * QC) {
+            int ii, pp;
+            if (!tL) { ii = e % MR; pp = e / MR; } else { pp = e % QC; ii = e / QC; }
+            Lb[pp * LP + ii] = lr[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < RN; ++u) {
+        const int e = tid + 256 * u;
+        if (e < QC * NC) {
+            int jj, pp;
+            if (!tR) { pp = e % QC; jj = e / QC; } else { jj = e % NC; pp = e / NC; }
+            Rb[pp * RP + jj] = a * rr[u];
+        }
+    }
+}
+
+template <typename T, int MR, int NC, int TM, int TN, int ROLE>
 __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restrict__ tiles,
                                                    const Contrib* __restrict__ cs, T alpha, T beta)
 {
     constexpr int QC = TileCfg<T>::QC;
-    constexpr int GS = 256 / MR;                // column groups
-    constexpr int LN = (MR * QC + 255) / 256;   // L elements staged per thread
-    constexpr int RN = (QC * 64 + 255) / 256;   // R elements staged per thread
-    __shared__ T Ls[QC][MR + 1];
-    __shared__ T Rs[QC][65];
+    constexpr int RT = MR / TM;
+    static_assert(RT * (NC / TN) == 256, "the thread grid must cover 256 threads");
+    constexpr int LN = (MR * QC + 255) / 256;
+    constexpr int RN = (QC * NC + 255) / 256;
+    constexpr int LP = MR + 4, RP = NC + 4;     // padded LDS rows (16-byte multiples)
+    __shared__ T Ls[2][QC * LP];
+    __shared__ T Rs[2][QC * RP];
     const Tile tl = tiles[blockIdx.x];
     int m = tl.m, k = tl.k;
     if (tl.mrank >= 0) m = min(m, max(0, b.rank[tl.mslot][tl.mrank] - tl.moff));
     if (tl.krank >= 0) k = min(k, max(0, b.rank[tl.kslot][tl.krank] - tl.koff));
     if (m <= 0 || k <= 0) return;
-    const int tid = threadIdx.x, i = tid % MR, g = tid / MR;
-    T acc[KPT];
+    const int tid = threadIdx.x, ti = tid % RT, tj = tid / RT;
+    T acc[TM][TN];
 #pragma unroll
-    for (int kk = 0; kk < KPT; ++kk) acc[kk] = T(0);
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = T(0);
 
-    for (int c = tl.cb; c < tl.ce; ++c) {
-        const Contrib C = cs[c];
-        int q = C.q;
-        if (C.qrank >= 0) q = min(q, b.rank[C.qslot][C.qrank]);
-        if (q <= 0) continue;
-        const T* __restrict__ L = reinterpret_cast<const T*>(b.val[C.lslot]) + C.loff;
-        const T* __restrict__ R = reinterpret_cast<const T*>(b.val[C.rslot]) + C.roff;
-        const T a = (C.flags & CF_RUNTIME_ALPHA) ? alpha * T(C.alpha)
-                  : (C.flags & CF_RUNTIME_BETA) ? beta * T(C.alpha) : T(C.alpha);
-        const bool tL = C.flags & CF_TRANS_L, tR = C.flags & CF_TRANS_R;
-        for (int p0 = 0; p0 < q; p0 += QC) {
-            const int qc = min(QC, q - p0);
-            T lr[LN], rr[RN];
-            // L(ii, pp): non-transposed walks ii fastest, transposed walks pp fastest.
-#pragma unroll
-            for (int u = 0; u < LN; ++u) {
-                const int e = tid + 256 * u;
-                int ii, pp;
-                if (!tL) { ii = e % MR; pp = e / MR; } else { pp = e % QC; ii = e / QC; }
-                const bool ok = e < MR * QC && ii < m && pp < qc;
-                lr[u] = ok ? (!tL ? L[ii + (int64_t)(p0 + pp) * C.ldl] : L[(p0 + pp) + (int64_t)ii * C.ldl]) : T(0);
-            }
-#pragma unroll
-            for (int u = 0; u < RN; ++u) {
-                const int e = tid + 256 * u;
-                int jj, pp;
-                if (!tR) { pp = e % QC; jj = e / QC; } else { jj = e % 64; pp = e / 64; }
-                const bool ok = e < QC * 64 && jj < k && pp < qc;
-                rr[u] = ok ? (!tR ? R[(p0 + pp) + (int64_t)jj * C.ldr] : R[jj + (int64_t)(p0 + pp) * C.ldr]) : T(0);
-            }
-#pragma unroll
-            for (int u = 0; u < LN; ++u) {
-                const int e = tid + 256 * u;
-                if (e < MR * QC) {
-                    int ii, pp;
-                    if (!tL) { ii = e % MR; pp = e / MR; } else { pp = e % QC; ii = e / QC; }
-                    Ls[pp][ii] = lr[u];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < RN; ++u) {
-                const int e = tid + 256 * u;
-                if (e < QC * 64) {
-                    int jj, pp;
-                    if (!tR) { pp = e % QC; jj = e / QC; } else { jj = e % 64; pp = e / 64; }
-                    Rs[pp][jj] = a * rr[u];
-                }
-            }
-            __syncthreads();
-            for (int p = 0; p < qc; ++p) {
-                const T l = Ls[p][i];
-#pragma unroll
-                for (int kk = 0; kk < KPT; ++kk) acc[kk] += l * Rs[p][g + GS * kk];
-            }
-            __syncthreads();
-        }
+    // Chunk iterator over (contribution c, offset p0 < q); uniform across the workgroup.
+    int c = tl.cb, q = 0, p0 = 0;
+    for (; c < tl.ce; ++c) {
+        q = contrib_q(b, cs[c]);
+        if (q > 0) break;
     }
-    if (i < m) {
-        T* out = reinterpret_cast<T*>(b.val[tl.slot]) + tl.off;
-        const bool useBeta = (tl.flags & TF_RUNTIME_BETA) && beta != T(0);
-        const bool accum = (tl.flags & TF_ACCUMULATE);
+    T lr[LN], rr[RN], a = T(0);
+    uint8_t flags = 0;
+    if (c < tl.ce) {
+        const Contrib C = cs[c];
+        tile_load<T, MR, NC, QC, LN, RN>(b, C, p0, q, m, k, alpha, beta, lr, rr, a);
+        flags = C.flags;
+    }
+    int buf = 0;
+    while (c < tl.ce) {
+        tile_store<T, MR, NC, QC, LN, RN, LP, RP>(Ls[buf], Rs[buf], lr, rr, a, flags);
+        __syncthreads();
+        // Advance and prefetch the next chunk while this one is multiplied.
+        p0 += QC;
+        if (p0 >= q) {
+            p0 = 0;
+            for (++c; c < tl.ce; ++c) {
+                q = contrib_q(b, cs[c]);
+                if (q > 0) break;
+            }
+        }
+        if (c < tl.ce) {
+            const Contrib C = cs[c];
+            tile_load<T, MR, NC, QC, LN, RN>(b, C, p0, q, m, k, alpha, beta, lr, rr, a);
+            flags = C.flags;
+        }
+        const T* __restrict__ Lb = Ls[buf];
+        const T* __restrict__ Rb = Rs[buf];
+#pragma unroll 4
+        for (int p = 0; p < QC; ++p) {
+            T lv[TM], rv[TN];
 #pragma unroll
-        for (int kk = 0; kk < KPT; ++kk) {
-            const int j = g + GS * kk;
-            if (j < k) {
-                T* o = out + i + (int64_t)j * tl.ld;
-                if (useBeta) *o = beta * *o + acc[kk];
-                else if (accum) *o += acc[kk];
-                else *o = acc[kk];
+            for (int i = 0; i < TM; ++i) lv[i] = Lb[p * LP + ti * TM + i];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) rv[j] = Rb[p * RP + tj * TN + j];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) acc[i][j] += lv[i] * rv[j];
+        }
+        buf ^= 1;
+    }
+    T* out = reinterpret_cast<T*>(b.val[tl.slot]) + tl.off;
+    const bool useBeta = (tl.flags & TF_RUNTIME_BETA) && beta != T(0);
+    const bool accum = (tl.flags & TF_ACCUMULATE);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+        const int r = ti * TM + i;
+        if (r >= m) continue;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int col = tj * TN + j;
+            if (col < k) {
+                T* o = out + r + (int64_t)col * tl.ld;
+                if (useBeta) *o = beta * *o + acc[i][j];
+                else if (accum) *o += acc[i][j];
+                else *o = acc[i][j];
             }
         }
     }
@@ -132,13 +195,10 @@ template <typename T, int ROLE, int MR>
 static void launch_tile_mr(int n, int kmax, const Bases& b, const Tile* t, const Contrib* c, T alpha, T beta,
                            hipStream_t s)
 {
-    constexpr int GS = 256 / MR;
-    const int kpt = (kmax + GS - 1) / GS;
-    if (kpt <= 1) tile_kernel<T, 1, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-    else if (kpt <= 2) tile_kernel<T, 2, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-    else if (kpt <= 4) tile_kernel<T, 4, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-    else if (kpt <= 8) tile_kernel<T, 8, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
-    else tile_kernel<T, 16, ROLE, MR><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    constexpr int TM = MR == 64 ? 4 : 1;
+    if (kmax <= 16) tile_kernel<T, MR, 16, TM, 1, ROLE><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else if (kmax <= 32) tile_kernel<T, MR, 32, TM, 2, ROLE><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
+    else tile_kernel<T, MR, 64, TM, 4, ROLE><<<n, 256, 0, s>>>(b, t, c, alpha, beta);
 }
 
 template <typename T>
@@ -149,6 +209,7 @@ void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStre
     const Tile* t = static_cast<const Tile*>(ph.dTiles);
     const Contrib* c = static_cast<const Contrib*>(ph.dContribs);
     const int kmax = ph.maxKTile;
+    if (kmax > 64 || ph.maxMTile > 64) throw std::runtime_error("tile larger than 64 x 64");
     // ROLE only distinguishes projection and accumulation launches in profiles;
     // MR = 16 serves phases whose tiles all have at most 16 rows.
     if (ph.maxMTile <= 16) {
@@ -190,56 +251,113 @@ __device__ inline double bcast(double v, int l)
 
 // ---------------------------------------------------------------------------
 // Streaming H*X (K right-hand sides, HBM-bound).
-// mv_proj: one wave per (low-rank block, chunk of <= 512 rows):
-//   P(j, kk) (+)= sum_r V(r, j) X(r, kk); lanes stream consecutive rows of V
-//   (coalesced), partial sums reduced across the wave at the end.
-// mv_acc: one workgroup per output row atom; its 4 waves split the atom's
-//   contributions (dense leaves D X, low-rank U P); lane i owns row i and reads
-//   L(i, p) coalesced along i; the R column is held one value per lane and
-//   broadcast with readlane; one LDS reduction across the waves at the end.
+// Both kernels read the factor panels with 16-byte vector loads: a lane owns
+// VW = 16 / sizeof(T) consecutive rows (a "row group"), LPC = 64 / VW lanes
+// cover 64 rows of one column, and the CG = VW lane groups of a wave walk
+// different columns, so one load instruction moves 1 KiB of a 64-row panel.
+// Panels whose offset or leading dimension is not a multiple of VW (ragged
+// grids) take the same lane mapping with scalar loads.
+//
+// mv_proj: one wave per (admissible block, <= 512-row chunk of its source
+//   box): P(j, kk) (+)= sum_r V(r, j) X(r, kk). Lane (rg, cg) accumulates its
+//   rows of columns cg + CG*u in registers; one butterfly over the LPC lanes of
+//   a column group finishes each sum (atomic add when several chunks share P).
+// mv_acc: one workgroup per output row atom (<= 64 rows); its 4 waves split
+//   the atom's contributions (dense leaves D X, low-rank U P). Per batch of
+//   up to CG*16 columns a wave stages the matching R entries in LDS, issues
+//   all 16 vector loads of L, then accumulates; column groups and waves are
+//   reduced at the end (shuffles, then LDS).
 // ---------------------------------------------------------------------------
+template <typename T> struct VecOf;
+template <> struct VecOf<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct VecOf<double> { typedef double type __attribute__((ext_vector_type(2))); };
+
 template <typename T, int K>
 __global__ void __launch_bounds__(256) mv_proj_kernel(Bases b, const MvProjItem* __restrict__ items, int n,
                                                       int ldx)
 {
+    constexpr int VW = 16 / (int)sizeof(T);
+    constexpr int LPC = 64 / VW;                // lanes per column
+    constexpr int CG = VW;                      // column groups per wave
+    constexpr int UN = 16 / CG;                 // columns per lane per pass (16 columns per pass)
+    typedef typename VecOf<T>::type vec;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int id = blockIdx.x * 4 + w;
     if (id >= n) return;
     const MvProjItem it = items[id];
     const int rank = min(it.cap, b.rank[SLOT_A][it.rankIdx]);
     if (rank <= 0) return;
+    const int rg = lane % LPC, cg = lane / LPC;
     const T* __restrict__ V = reinterpret_cast<const T*>(b.val[SLOT_A]) + it.voff;
     const T* __restrict__ X = reinterpret_cast<const T*>(b.val[SLOT_X]) + it.xrow;
     T* P = reinterpret_cast<T*>(b.val[SLOT_WS]) + it.poff;
+    const bool vecV = (it.voff % VW == 0) && (it.ldv % VW == 0);
+    const bool vecX = (it.xrow % VW == 0) && (ldx % VW == 0) &&
+                      ((reinterpret_cast<uintptr_t>(b.val[SLOT_X]) & 15) == 0);
     for (int j0 = 0; j0 < rank; j0 += 16) {
-        T acc[16][K];
+        T acc[UN][K];
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj)
+        for (int u = 0; u < UN; ++u)
 #pragma unroll
-            for (int kk = 0; kk < K; ++kk) acc[jj][kk] = T(0);
-        for (int r = lane; r < it.rows; r += 64) {
-            T xv[K];
-#pragma unroll
-            for (int kk = 0; kk < K; ++kk) xv[kk] = X[r + (int64_t)kk * ldx];
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                if (j0 + jj < rank) {
-                    const T v = V[r + (int64_t)(j0 + jj) * it.ldv];
-#pragma unroll
-                    for (int kk = 0; kk < K; ++kk) acc[jj][kk] += v * xv[kk];
-                }
-            }
-        }
-#pragma unroll
-        for (int jj = 0; jj < 16; ++jj) {
-            if (j0 + jj >= rank) break;
+            for (int kk = 0; kk < K; ++kk) acc[u][kk] = T(0);
+        for (int r0 = rg * VW; r0 < it.rows; r0 += 64) {
+            const bool full = r0 + VW <= it.rows;
+            T xv[K][VW];
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) {
-                const T sum = wave_sum(acc[jj][kk]);
-                if (lane == 0) {
-                    T* dst = P + (j0 + jj) + (int64_t)kk * it.cap;
-                    if (it.atomic) atomicAdd(dst, sum);
-                    else *dst = sum;
+                if (full && vecX) {
+                    const vec v = *reinterpret_cast<const vec*>(X + r0 + (int64_t)kk * ldx);
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) xv[kk][e] = v[e];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VW; ++e)
+                        xv[kk][e] = (r0 + e < it.rows) ? X[r0 + e + (int64_t)kk * ldx] : T(0);
+                }
+            }
+            vec lv[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                const int j = j0 + cg + CG * u;
+                if (j < rank) {
+                    const T* col = V + r0 + (int64_t)j * it.ldv;
+                    if (full && vecV) lv[u] = *reinterpret_cast<const vec*>(col);
+                    else {
+#pragma unroll
+                        for (int e = 0; e < VW; ++e) lv[u][e] = (r0 + e < it.rows) ? col[e] : T(0);
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) lv[u][e] = T(0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UN; ++u)
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk)
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) acc[u][kk] += lv[u][e] * xv[kk][e];
+        }
+        // Sum over the LPC lanes of each column group (lane bits below LPC).
+#pragma unroll
+        for (int u = 0; u < UN; ++u)
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) {
+                T v = acc[u][kk];
+#pragma unroll
+                for (int off = LPC / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+                acc[u][kk] = v;
+            }
+        if (rg == 0) {
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                const int j = j0 + cg + CG * u;
+                if (j >= rank) continue;
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) {
+                    T* dst = P + j + (int64_t)kk * it.cap;
+                    if (it.atomic) atomicAdd(dst, acc[u][kk]);
+                    else *dst = acc[u][kk];
                 }
             }
         }
@@ -251,13 +369,24 @@ __global__ void __launch_bounds__(256) mv_acc_kernel(Bases b, const MvAtom* __re
                                                      const MvContrib* __restrict__ cs, int ldx, int ldy,
                                                      T alpha, T beta)
 {
+    constexpr int VW = 16 / (int)sizeof(T);
+    constexpr int LPC = 64 / VW;
+    constexpr int CG = VW;
+    constexpr int UN = 16;                      // vector loads in flight per lane
+    constexpr int BC = CG * UN;                 // columns per batch
+    typedef typename VecOf<T>::type vec;
+    __shared__ T xs[4][K][BC];
     __shared__ T red[4][K][64];
     const MvAtom at = atoms[blockIdx.x];
-    const int w = threadIdx.x >> 6, i = threadIdx.x & 63;
-    const bool row = i < at.m;
-    T acc[K];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rg = lane % LPC, cg = lane / LPC;
+    const int r0 = rg * VW;
+    const bool full = r0 + VW <= at.m;
+    T acc[VW][K];
 #pragma unroll
-    for (int kk = 0; kk < K; ++kk) acc[kk] = T(0);
+    for (int e = 0; e < VW; ++e)
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) acc[e][kk] = T(0);
     const T* __restrict__ H = reinterpret_cast<const T*>(b.val[SLOT_A]);
     const T* __restrict__ X = reinterpret_cast<const T*>(b.val[SLOT_X]);
     const T* __restrict__ WS = reinterpret_cast<const T*>(b.val[SLOT_WS]);
@@ -268,35 +397,60 @@ __global__ void __launch_bounds__(256) mv_acc_kernel(Bases b, const MvAtom* __re
         const T* __restrict__ L = H + C.loff;
         const T* __restrict__ R = (C.kind == 0 ? X : WS) + C.roff;
         const int ldr = C.kind == 0 ? ldx : C.ldr;
-        for (int p0 = 0; p0 < q; p0 += 64) {
-            const int qc = min(64, q - p0);
-            T rv[K];
+        const bool vecL = full && (C.loff % VW == 0) && (C.ld % VW == 0);
+        for (int p0 = 0; p0 < q; p0 += BC) {
+            const int qc = min(BC, q - p0);
+            // Stage R(p0 : p0 + qc, :) for this wave.
 #pragma unroll
-            for (int kk = 0; kk < K; ++kk) rv[kk] = (i < qc) ? R[p0 + i + (int64_t)kk * ldr] : T(0);
-            const T* __restrict__ Lp = L + i + (int64_t)p0 * C.ld;
-            int p = 0;
-            for (; p + 16 <= qc; p += 16) {
-                T l[16];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) l[u] = row ? Lp[(int64_t)(p + u) * C.ld] : T(0);
-#pragma unroll
-                for (int u = 0; u < 16; ++u)
-#pragma unroll
-                    for (int kk = 0; kk < K; ++kk) acc[kk] += l[u] * bcast(rv[kk], p + u);
+            for (int e = lane; e < BC * K; e += 64) {
+                const int kk = e / BC, j = e % BC;
+                xs[w][kk][j] = (j < qc) ? R[p0 + j + (int64_t)kk * ldr] : T(0);
             }
-            for (; p < qc; ++p) {
-                const T l = row ? Lp[(int64_t)p * C.ld] : T(0);
+            __builtin_amdgcn_wave_barrier();
+            vec lv[UN];
 #pragma unroll
-                for (int kk = 0; kk < K; ++kk) acc[kk] += l * bcast(rv[kk], p);
+            for (int u = 0; u < UN; ++u) {
+                const int j = cg + CG * u;
+                const T* col = L + r0 + (int64_t)(p0 + j) * C.ld;
+                if (j < qc && vecL) lv[u] = *reinterpret_cast<const vec*>(col);
+                else {
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) lv[u][e] = (j < qc && r0 + e < at.m) ? col[e] : T(0);
+                }
             }
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                const int j = cg + CG * u;
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) {
+                    const T xv = xs[w][kk][j];
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) acc[e][kk] += lv[u][e] * xv;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
+    // Sum the column groups of the wave (lane bits at and above LPC).
 #pragma unroll
-    for (int kk = 0; kk < K; ++kk) red[w][kk][i] = acc[kk];
-    __syncthreads();
-    if (w == 0 && row) {
+    for (int e = 0; e < VW; ++e)
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
+            T v = acc[e][kk];
+#pragma unroll
+            for (int off = LPC; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+            acc[e][kk] = v;
+        }
+    if (cg == 0) {
+#pragma unroll
+        for (int e = 0; e < VW; ++e)
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) red[w][kk][r0 + e] = acc[e][kk];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * K; e += 256) {
+        const int i = e & 63, kk = e >> 6;
+        if (i < at.m) {
             const T sum = red[0][kk][i] + red[1][kk][i] + red[2][kk][i] + red[3][kk][i];
             T* y = reinterpret_cast<T*>(b.val[SLOT_Y]) + at.yrow + i + (int64_t)kk * ldy;
             *y = (beta != T(0) ? beta * *y : T(0)) + alpha * sum;
@@ -451,12 +605,95 @@ __global__ void __launch_bounds__(256) qr_kernel(Bases b, const QrTask* __restri
     }
 }
 
+// One wave per panel with m <= 64 rows and k <= KMAX columns: lane i holds
+// row i of the panel in registers (columns unrolled at compile time), so a
+// Householder step is one wave reduction per remaining column and no
+// barriers. FP64 arithmetic; same reflector convention and outputs (Q in
+// place, R to the workspace) as qr_kernel.
+template <typename T, int KMAX>
+__global__ void __launch_bounds__(256) qr_wave_kernel(Bases b, const QrTask* __restrict__ tasks, int n)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int id = blockIdx.x * 4 + w;
+    if (id >= n) return;
+    const QrTask tk = tasks[id];
+    T* A = reinterpret_cast<T*>(b.val[tk.slot]) + tk.off;
+    const int m = tk.m, k = tk.k, kk = min(m, k);
+    if (m <= 0) return;
+    const bool row = lane < m;
+    double a[KMAX], tau[KMAX];
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) {
+        a[c] = (row && c < k) ? (double)A[lane + (int64_t)c * m] : 0.0;
+        tau[c] = 0.0;
+    }
+    // Factor: reflector j zeroes column j below the diagonal.
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        if (j >= kk) continue;
+        const double x = a[j];
+        const double xn2 = wave_sum((row && lane > j) ? x * x : 0.0);
+        const double alpha = bcast(x, j);
+        double t = 0.0, scale = 0.0, beta = alpha;
+        if (xn2 > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+            t = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        const double v = (row && lane > j) ? x * scale : (lane == j ? 1.0 : 0.0);
+#pragma unroll
+        for (int c = j + 1; c < KMAX; ++c) {
+            if (c < k) {
+                const double d = wave_sum(v * a[c]);
+                a[c] -= t * d * v;
+            }
+        }
+        tau[j] = t;
+        a[j] = (lane == j) ? beta : ((lane > j) ? v : x);
+    }
+    if (tk.roff >= 0) {
+        T* R = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.roff;
+        if (lane < k) {
+#pragma unroll
+            for (int c = 0; c < KMAX; ++c)
+                if (c < k) R[lane + c * k] = (lane <= c && lane < kk) ? (T)a[c] : T(0);
+        }
+    }
+    // Q = H_0 ... H_{kk-1} [I; 0], accumulated backwards (orgqr).
+    double q[KMAX];
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) q[c] = (lane == c && c < kk) ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = KMAX - 1; j >= 0; --j) {
+        if (j >= kk) continue;
+        const double v = (lane == j) ? 1.0 : ((row && lane > j) ? a[j] : 0.0);
+#pragma unroll
+        for (int c = j; c < KMAX; ++c) {
+            if (c < k) {
+                const double d = wave_sum(v * q[c]);
+                q[c] -= tau[j] * d * v;
+            }
+        }
+    }
+    if (row) {
+#pragma unroll
+        for (int c = 0; c < KMAX; ++c)
+            if (c < k) A[lane + (int64_t)c * m] = (T)q[c];
+    }
+}
+
 template <typename T>
 void launch_qr_phase(const Phase& ph, const Bases& b, hipStream_t s)
 {
     const int n = (int)ph.qr.size();
     if (n == 0) return;
-    qr_kernel<T><<<n, 256, 0, s>>>(b, static_cast<const QrTask*>(ph.dTasks));
+    const QrTask* tasks = static_cast<const QrTask*>(ph.dTasks);
+    const int ns = ph.qrSmall;
+    if (ns > 0) {
+        if (ph.qrSmallK <= 24) qr_wave_kernel<T, 24><<<(ns + 3) / 4, 256, 0, s>>>(b, tasks, ns);
+        else qr_wave_kernel<T, 40><<<(ns + 3) / 4, 256, 0, s>>>(b, tasks, ns);
+    }
+    if (n > ns) qr_kernel<T><<<n - ns, 256, 0, s>>>(b, tasks + ns);
     HIP_LAUNCH_CHECK();
 }
 

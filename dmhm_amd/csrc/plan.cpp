@@ -191,6 +191,9 @@ Plan* Planner::matvecStream(const Layout& L, int k, int ldx, int ldy)
         }
         plan->flops += 2.0 * sb.size * L.cap[i] * k;
     }
+    // Longest chunks first: the grid's tail is then made of short items.
+    std::stable_sort(pr.mvProj.begin(), pr.mvProj.end(),
+                     [](const MvProjItem& a, const MvProjItem& b) { return a.rows > b.rows; });
     // Contributions per output atom.
     std::vector<std::vector<MvContrib>> per(g.atomStart.size());
     for (int b : g.leaves) {
@@ -212,7 +215,11 @@ Plan* Planner::matvecStream(const Layout& L, int k, int ldx, int ldy)
             per[a].push_back(c);
         }
     }
-    for (size_t a = 0; a < per.size(); ++a) {
+    // Atoms with the most contributions first (same tail argument).
+    std::vector<int> order(per.size());
+    for (size_t a = 0; a < per.size(); ++a) order[a] = (int)a;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return per[x].size() > per[y].size(); });
+    for (int a : order) {
         MvAtom at{};
         at.yrow = g.atomStart[a]; at.m = g.atomSize[a];
         at.cb = (int)ac.mvContribs.size();

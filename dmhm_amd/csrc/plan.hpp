@@ -115,6 +115,8 @@ struct Phase {
     int mvK = 0, ldx = 0, ldy = 0;
     void* dAux = nullptr;
     int qrK = 0;            // panel width of the QR phase
+    int qrSmall = 0;        // tasks [0, qrSmall) fit one wave (m <= 64, k <= 40), set by upload
+    int qrSmallK = 0;       // widest of those tasks
     // device copies (filled by upload)
     void* dTiles = nullptr; void* dContribs = nullptr; void* dTasks = nullptr;
     int maxKTile = 0;       // widest tile (selects the kernel variant)

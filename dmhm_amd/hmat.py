@@ -315,6 +315,20 @@ class _HMatBase:
                                           fl.ctypes.data_as(dp)))
         return ms, fl
 
+    @staticmethod
+    def last_hh_phases(cap=64):
+        """Per-phase (kind, ms, flops) of the last profile_hh call."""
+        ms = np.zeros(cap)
+        fl = np.zeros(cap)
+        kinds = np.zeros(cap, dtype=np.int32)
+        cnt = ctypes.c_int()
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(_lib.load().dmhm_profile_hh_phases(cap, ms.ctypes.data_as(dp), fl.ctypes.data_as(dp),
+                                                 kinds.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                 ctypes.byref(cnt)))
+        n = min(cnt.value, cap)
+        return [(int(kinds[i]), float(ms[i]), float(fl[i])) for i in range(n)]
+
     def to_dense(self):
         n = self.Height()
         out = np.zeros((n, n), order="F")

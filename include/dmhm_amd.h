@@ -250,6 +250,12 @@ int dmhm_profile_matvec(dmhm_hmat h, int k, int reps, double* ms, double* bytes)
 int dmhm_profile_hh(double alpha, dmhm_hmat A, dmhm_hmat B, const dmhm_hh_opts* opts, double* ms,
                     double* flops);
 
+/* Per-phase timings of the last dmhm_profile_hh call (measurement support,
+ * no counterpart in the reference): phase kind (0 tile, 1 QR, 2 SVD, 3 RNG),
+ * milliseconds and algorithmic flops, in plan order; *count = number of
+ * phases (at most `cap` entries written). */
+int dmhm_profile_hh_phases(int cap, double* ms, double* flops, int* kinds, int* count);
+
 #ifdef __cplusplus
 }
 #endif
