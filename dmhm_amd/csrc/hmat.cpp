@@ -1466,15 +1466,34 @@ void Plan::upload()
         }
         const void* src = nullptr; size_t bytes = 0;
         if (ph.kind == PH_QR && !ph.qr.empty()) {
-            // Panels that fit one wave first (independent tasks; order is free).
-            auto small = [](const QrTask& t) { return t.m <= 64 && t.k <= 40; };
-            auto mid = std::stable_partition(ph.qr.begin(), ph.qr.end(), small);
-            ph.qrSmall = (int)(mid - ph.qr.begin());
-            ph.qrSmallK = 0;
-            for (auto it = ph.qr.begin(); it != mid; ++it) ph.qrSmallK = std::max(ph.qrSmallK, it->k);
+            // Sort the (independent) tasks by panel class: register-resident
+            // kernels for m <= 64 / 256 / 512 / 1024 with k <= 40, then the rest.
+            auto cls = [](const QrTask& t) {
+                if (t.k > 40) return 4;
+                if (t.m <= 64) return 0;
+                if (t.m <= 256) return 1;
+                if (t.m <= 512) return 2;
+                return (t.m <= 1024 && t.k <= 24) ? 3 : 4;
+            };
+            std::stable_sort(ph.qr.begin(), ph.qr.end(),
+                             [&](const QrTask& x, const QrTask& y) { return cls(x) < cls(y); });
+            for (int c = 0; c < 4; ++c) ph.qrClassEnd[c] = 0;
+            ph.qrRegK = 0;
+            int pos = 0;
+            for (int c = 0; c < 4; ++c) {
+                while (pos < (int)ph.qr.size() && cls(ph.qr[pos]) == c) {
+                    if (c < 3) ph.qrRegK = std::max(ph.qrRegK, ph.qr[pos].k);
+                    ++pos;
+                }
+                ph.qrClassEnd[c] = pos;
+            }
             src = ph.qr.data(); bytes = ph.qr.size() * sizeof(QrTask);
         }
-        if (ph.kind == PH_SVD && !ph.svd.empty()) { src = ph.svd.data(); bytes = ph.svd.size() * sizeof(SvdTask); }
+        if (ph.kind == PH_SVD && !ph.svd.empty()) {
+            ph.svdK = 0;
+            for (const auto& t : ph.svd) ph.svdK = std::max(ph.svdK, t.k);
+            src = ph.svd.data(); bytes = ph.svd.size() * sizeof(SvdTask);
+        }
         if (ph.kind == PH_RNG && !ph.rng.empty()) { src = ph.rng.data(); bytes = ph.rng.size() * sizeof(RngTask); }
         if (ph.kind == PH_MV_PROJ && !ph.mvProj.empty()) {
             src = ph.mvProj.data(); bytes = ph.mvProj.size() * sizeof(MvProjItem);

@@ -9,6 +9,7 @@
 //  rng_kernel    Gaussian sketch panels (counter-based, order independent)
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include "kernels.hpp"
@@ -251,22 +252,21 @@ __device__ inline double bcast(double v, int l)
 
 // ---------------------------------------------------------------------------
 // Streaming H*X (K right-hand sides, HBM-bound).
-// Both kernels read the factor panels with 16-byte vector loads: a lane owns
-// VW = 16 / sizeof(T) consecutive rows (a "row group"), LPC = 64 / VW lanes
-// cover 64 rows of one column, and the CG = VW lane groups of a wave walk
-// different columns, so one load instruction moves 1 KiB of a 64-row panel.
-// Panels whose offset or leading dimension is not a multiple of VW (ragged
-// grids) take the same lane mapping with scalar loads.
-//
 // mv_proj: one wave per (admissible block, <= 512-row chunk of its source
-//   box): P(j, kk) (+)= sum_r V(r, j) X(r, kk). Lane (rg, cg) accumulates its
-//   rows of columns cg + CG*u in registers; one butterfly over the LPC lanes of
-//   a column group finishes each sum (atomic add when several chunks share P).
+//   box): P(j, kk) (+)= sum_r V(r, j) X(r, kk). V is read with 16-byte vector
+//   loads: a lane owns VW = 16 / sizeof(T) consecutive rows, LPC = 64 / VW
+//   lanes cover 64 rows of one column and the CG = VW lane groups of a wave
+//   walk different columns (1 KiB per load instruction); one butterfly over
+//   the LPC lanes of a column group finishes each sum (atomic add when several
+//   chunks share P). Panels whose offset or leading dimension is not a
+//   multiple of VW (ragged grids) use the same mapping with scalar loads.
 // mv_acc: one workgroup per output row atom (<= 64 rows); its 4 waves split
-//   the atom's contributions (dense leaves D X, low-rank U P). Per batch of
-//   up to CG*16 columns a wave stages the matching R entries in LDS, issues
-//   all 16 vector loads of L, then accumulates; column groups and waves are
-//   reduced at the end (shuffles, then LDS).
+//   the atom's contributions (dense leaves D X, low-rank U P). Lane i owns row
+//   i and issues 16 column loads per batch (each instruction moves one 64-row
+//   column segment); the R column is held one value per lane and broadcast
+//   with readlane; one LDS reduction across the waves at the end. (A variant
+//   with 16-byte row-group loads measured slower here: 143 us against 93 us
+//   on the 256^2 rank-16 operator.)
 // ---------------------------------------------------------------------------
 template <typename T> struct VecOf;
 template <> struct VecOf<float> { typedef float type __attribute__((ext_vector_type(4))); };
@@ -366,27 +366,17 @@ __global__ void __launch_bounds__(256) mv_proj_kernel(Bases b, const MvProjItem*
 
 template <typename T, int K>
 __global__ void __launch_bounds__(256) mv_acc_kernel(Bases b, const MvAtom* __restrict__ atoms,
-                                                     const MvContrib* __restrict__ cs, int ldx, int ldy,
-                                                     T alpha, T beta)
+                                                          const MvContrib* __restrict__ cs, int ldx, int ldy,
+                                                          T alpha, T beta)
 {
-    constexpr int VW = 16 / (int)sizeof(T);
-    constexpr int LPC = 64 / VW;
-    constexpr int CG = VW;
-    constexpr int UN = 16;                      // vector loads in flight per lane
-    constexpr int BC = CG * UN;                 // columns per batch
-    typedef typename VecOf<T>::type vec;
-    __shared__ T xs[4][K][BC];
+    constexpr int UNR = 16;
     __shared__ T red[4][K][64];
     const MvAtom at = atoms[blockIdx.x];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rg = lane % LPC, cg = lane / LPC;
-    const int r0 = rg * VW;
-    const bool full = r0 + VW <= at.m;
-    T acc[VW][K];
+    const int w = threadIdx.x >> 6, i = threadIdx.x & 63;
+    const bool row = i < at.m;
+    T acc[K];
 #pragma unroll
-    for (int e = 0; e < VW; ++e)
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk) acc[e][kk] = T(0);
+    for (int kk = 0; kk < K; ++kk) acc[kk] = T(0);
     const T* __restrict__ H = reinterpret_cast<const T*>(b.val[SLOT_A]);
     const T* __restrict__ X = reinterpret_cast<const T*>(b.val[SLOT_X]);
     const T* __restrict__ WS = reinterpret_cast<const T*>(b.val[SLOT_WS]);
@@ -397,60 +387,35 @@ __global__ void __launch_bounds__(256) mv_acc_kernel(Bases b, const MvAtom* __re
         const T* __restrict__ L = H + C.loff;
         const T* __restrict__ R = (C.kind == 0 ? X : WS) + C.roff;
         const int ldr = C.kind == 0 ? ldx : C.ldr;
-        const bool vecL = full && (C.loff % VW == 0) && (C.ld % VW == 0);
-        for (int p0 = 0; p0 < q; p0 += BC) {
-            const int qc = min(BC, q - p0);
-            // Stage R(p0 : p0 + qc, :) for this wave.
+        for (int p0 = 0; p0 < q; p0 += 64) {
+            const int qc = min(64, q - p0);
+            T rv[K];
 #pragma unroll
-            for (int e = lane; e < BC * K; e += 64) {
-                const int kk = e / BC, j = e % BC;
-                xs[w][kk][j] = (j < qc) ? R[p0 + j + (int64_t)kk * ldr] : T(0);
+            for (int kk = 0; kk < K; ++kk) rv[kk] = (i < qc) ? R[p0 + i + (int64_t)kk * ldr] : T(0);
+            const T* __restrict__ Lp = L + i + (int64_t)p0 * C.ld;
+            int p = 0;
+            for (; p + UNR <= qc; p += UNR) {
+                T l[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) l[u] = row ? Lp[(int64_t)(p + u) * C.ld] : T(0);
+#pragma unroll
+                for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                    for (int kk = 0; kk < K; ++kk) acc[kk] += l[u] * bcast(rv[kk], p + u);
             }
-            __builtin_amdgcn_wave_barrier();
-            vec lv[UN];
+            for (; p < qc; ++p) {
+                const T l = row ? Lp[(int64_t)p * C.ld] : T(0);
 #pragma unroll
-            for (int u = 0; u < UN; ++u) {
-                const int j = cg + CG * u;
-                const T* col = L + r0 + (int64_t)(p0 + j) * C.ld;
-                if (j < qc && vecL) lv[u] = *reinterpret_cast<const vec*>(col);
-                else {
-#pragma unroll
-                    for (int e = 0; e < VW; ++e) lv[u][e] = (j < qc && r0 + e < at.m) ? col[e] : T(0);
-                }
+                for (int kk = 0; kk < K; ++kk) acc[kk] += l * bcast(rv[kk], p);
             }
-#pragma unroll
-            for (int u = 0; u < UN; ++u) {
-                const int j = cg + CG * u;
-#pragma unroll
-                for (int kk = 0; kk < K; ++kk) {
-                    const T xv = xs[w][kk][j];
-#pragma unroll
-                    for (int e = 0; e < VW; ++e) acc[e][kk] += lv[u][e] * xv;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
         }
     }
-    // Sum the column groups of the wave (lane bits at and above LPC).
 #pragma unroll
-    for (int e = 0; e < VW; ++e)
+    for (int kk = 0; kk < K; ++kk) red[w][kk][i] = acc[kk];
+    __syncthreads();
+    if (w == 0 && row) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            T v = acc[e][kk];
-#pragma unroll
-            for (int off = LPC; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-            acc[e][kk] = v;
-        }
-    if (cg == 0) {
-#pragma unroll
-        for (int e = 0; e < VW; ++e)
-#pragma unroll
-            for (int kk = 0; kk < K; ++kk) red[w][kk][r0 + e] = acc[e][kk];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * K; e += 256) {
-        const int i = e & 63, kk = e >> 6;
-        if (i < at.m) {
             const T sum = red[0][kk][i] + red[1][kk][i] + red[2][kk][i] + red[3][kk][i];
             T* y = reinterpret_cast<T*>(b.val[SLOT_Y]) + at.yrow + i + (int64_t)kk * ldy;
             *y = (beta != T(0) ? beta * *y : T(0)) + alpha * sum;
@@ -605,80 +570,187 @@ __global__ void __launch_bounds__(256) qr_kernel(Bases b, const QrTask* __restri
     }
 }
 
-// One wave per panel with m <= 64 rows and k <= KMAX columns: lane i holds
-// row i of the panel in registers (columns unrolled at compile time), so a
-// Householder step is one wave reduction per remaining column and no
-// barriers. FP64 arithmetic; same reflector convention and outputs (Q in
-// place, R to the workspace) as qr_kernel.
-template <typename T, int KMAX>
-__global__ void __launch_bounds__(256) qr_wave_kernel(Bases b, const QrTask* __restrict__ tasks, int n)
+// Register-resident Householder QR for panels of m <= 64*NW*RPL rows and
+// k <= KMAX columns: one workgroup of NW waves per panel, thread t holding
+// rows t + 64*NW*r (r < RPL) in registers (columns unrolled at compile
+// time). Each reflector step needs one reduction round: the squared norm of
+// the column below the diagonal and its products with the later columns are
+// reduced together (wave sums, then the NW wave partials through LDS), and
+// the pivot row is published through LDS in the same round, so a step costs
+// one barrier. Q is formed in place (orgqr order). FP64 arithmetic; same
+// reflector convention and outputs as qr_kernel.
+template <typename T, int KMAX, int NW, int RPL>
+__global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* __restrict__ tasks)
 {
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int id = blockIdx.x * 4 + w;
-    if (id >= n) return;
-    const QrTask tk = tasks[id];
+    constexpr int NT = 64 * NW;
+    __shared__ double part[2][NW][KMAX];
+    __shared__ double rowj[2][KMAX];
+    __shared__ double taus[KMAX];
+    const QrTask tk = tasks[blockIdx.x];
     T* A = reinterpret_cast<T*>(b.val[tk.slot]) + tk.off;
     const int m = tk.m, k = tk.k, kk = min(m, k);
-    if (m <= 0) return;
-    const bool row = lane < m;
-    double a[KMAX], tau[KMAX];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    double a[RPL][KMAX];
 #pragma unroll
-    for (int c = 0; c < KMAX; ++c) {
-        a[c] = (row && c < k) ? (double)A[lane + (int64_t)c * m] : 0.0;
-        tau[c] = 0.0;
+    for (int r = 0; r < RPL; ++r) {
+        const int i = tid + NT * r;
+#pragma unroll
+        for (int c = 0; c < KMAX; ++c) a[r][c] = (i < m && c < k) ? (double)A[i + (int64_t)c * m] : 0.0;
     }
-    // Factor: reflector j zeroes column j below the diagonal.
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
-        if (j >= kk) continue;
-        const double x = a[j];
-        const double xn2 = wave_sum((row && lane > j) ? x * x : 0.0);
-        const double alpha = bcast(x, j);
-        double t = 0.0, scale = 0.0, beta = alpha;
-        if (xn2 > 0.0) {
-            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-            t = (beta - alpha) / beta;
-            scale = 1.0 / (alpha - beta);
-        }
-        const double v = (row && lane > j) ? x * scale : (lane == j ? 1.0 : 0.0);
+        if (j < kk) {
+            const int buf = j & 1;
+            double p[KMAX];
 #pragma unroll
-        for (int c = j + 1; c < KMAX; ++c) {
-            if (c < k) {
-                const double d = wave_sum(v * a[c]);
-                a[c] -= t * d * v;
+            for (int c = j; c < KMAX; ++c) p[c] = 0.0;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int i = tid + NT * r;
+                if (i > j && i < m) {
+                    const double x = a[r][j];
+#pragma unroll
+                    for (int c = j; c < KMAX; ++c) p[c] += x * a[r][c];
+                }
+                if (i == j) {
+#pragma unroll
+                    for (int c = j; c < KMAX; ++c) rowj[buf][c] = a[r][c];
+                }
             }
+#pragma unroll
+            for (int c = j; c < KMAX; ++c) {
+                if (c < k) {
+                    const double sum = wave_sum(p[c]);
+                    if (lane == 0) part[buf][w][c] = sum;
+                }
+            }
+            __syncthreads();
+            double xn2 = 0.0;
+#pragma unroll
+            for (int q = 0; q < NW; ++q) xn2 += part[buf][q][j];
+            const double alpha = rowj[buf][j];
+            double t = 0.0, scale = 0.0, beta = alpha;
+            if (xn2 > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                t = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+#pragma unroll
+            for (int c = j + 1; c < KMAX; ++c) {
+                if (c < k) {
+                    double d = 0.0;
+#pragma unroll
+                    for (int q = 0; q < NW; ++q) d += part[buf][q][c];
+                    const double wc = t * (rowj[buf][c] + scale * d);
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) {
+                        const int i = tid + NT * r;
+                        if (i == j) a[r][c] -= wc;
+                        else if (i > j && i < m) a[r][c] -= wc * (a[r][j] * scale);
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int i = tid + NT * r;
+                if (i == j) a[r][j] = beta;
+                else if (i > j && i < m) a[r][j] *= scale;
+            }
+            if (tid == 0) taus[j] = t;
         }
-        tau[j] = t;
-        a[j] = (lane == j) ? beta : ((lane > j) ? v : x);
     }
+    __syncthreads();
     if (tk.roff >= 0) {
         T* R = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.roff;
-        if (lane < k) {
 #pragma unroll
-            for (int c = 0; c < KMAX; ++c)
-                if (c < k) R[lane + c * k] = (lane <= c && lane < kk) ? (T)a[c] : T(0);
-        }
-    }
-    // Q = H_0 ... H_{kk-1} [I; 0], accumulated backwards (orgqr).
-    double q[KMAX];
+        for (int r = 0; r < RPL; ++r) {
+            const int i = tid + NT * r;
+            if (i < k) {
 #pragma unroll
-    for (int c = 0; c < KMAX; ++c) q[c] = (lane == c && c < kk) ? 1.0 : 0.0;
-#pragma unroll
-    for (int j = KMAX - 1; j >= 0; --j) {
-        if (j >= kk) continue;
-        const double v = (lane == j) ? 1.0 : ((row && lane > j) ? a[j] : 0.0);
-#pragma unroll
-        for (int c = j; c < KMAX; ++c) {
-            if (c < k) {
-                const double d = wave_sum(v * q[c]);
-                q[c] -= tau[j] * d * v;
+                for (int c = 0; c < KMAX; ++c)
+                    if (c < k) R[i + c * k] = (i <= c && i < kk) ? (T)a[r][c] : T(0);
             }
         }
     }
-    if (row) {
+    // Q in place: columns >= kk start at zero, then H_j for j = kk-1 .. 0.
 #pragma unroll
-        for (int c = 0; c < KMAX; ++c)
-            if (c < k) A[lane + (int64_t)c * m] = (T)q[c];
+    for (int c = 0; c < KMAX; ++c)
+        if (c >= kk) {
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) a[r][c] = 0.0;
+        }
+#pragma unroll
+    for (int j = KMAX - 1; j >= 0; --j) {
+        if (j < kk) {
+            const int buf = j & 1;
+            const double t = taus[j];
+            double p[KMAX];
+#pragma unroll
+            for (int c = j + 1; c < KMAX; ++c) p[c] = 0.0;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int i = tid + NT * r;
+                const double v = (i == j) ? 1.0 : ((i > j && i < m) ? a[r][j] : 0.0);
+#pragma unroll
+                for (int c = j + 1; c < KMAX; ++c) p[c] += v * a[r][c];
+            }
+#pragma unroll
+            for (int c = j + 1; c < KMAX; ++c) {
+                if (c < k) {
+                    const double sum = wave_sum(p[c]);
+                    if (lane == 0) part[buf][w][c] = sum;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = j + 1; c < KMAX; ++c) {
+                if (c < k) {
+                    double d = 0.0;
+#pragma unroll
+                    for (int q = 0; q < NW; ++q) d += part[buf][q][c];
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) {
+                        const int i = tid + NT * r;
+                        const double v = (i == j) ? 1.0 : ((i > j && i < m) ? a[r][j] : 0.0);
+                        a[r][c] -= t * d * v;
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int i = tid + NT * r;
+                a[r][j] = (i == j) ? 1.0 - t : ((i > j && i < m) ? -t * a[r][j] : 0.0);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        const int i = tid + NT * r;
+        if (i < m) {
+#pragma unroll
+            for (int c = 0; c < KMAX; ++c)
+                if (c < k) A[i + (int64_t)c * m] = (T)a[r][c];
+        }
+    }
+}
+
+// Panel classes of a QR phase (tasks are sorted by class at upload):
+// 0: m <= 64, 1: m <= 256, 2: m <= 512 (k <= 40), 3: m <= 1024 with k <= 24
+// (16 waves of at most 128 registers fill a CU), 4: the rest (qr_kernel).
+template <typename T, int KMAX>
+static void launch_qr_classes(const QrTask* tasks, const int* classEnd, const Bases& b, hipStream_t s)
+{
+    int begin = 0;
+    for (int c = 0; c < 4; ++c) {
+        const int n = classEnd[c] - begin;
+        if (n > 0) {
+            const QrTask* t = tasks + begin;
+            if (c == 0) qr_reg_kernel<T, KMAX, 1, 1><<<n, 64, 0, s>>>(b, t);
+            else if (c == 1) qr_reg_kernel<T, KMAX, 4, 1><<<n, 256, 0, s>>>(b, t);
+            else if (c == 2) qr_reg_kernel<T, KMAX, 8, 1><<<n, 512, 0, s>>>(b, t);
+            else qr_reg_kernel<T, 24, 16, 1><<<n, 1024, 0, s>>>(b, t);    // class 3 has k <= 24
+        }
+        begin = classEnd[c];
     }
 }
 
@@ -688,12 +760,12 @@ void launch_qr_phase(const Phase& ph, const Bases& b, hipStream_t s)
     const int n = (int)ph.qr.size();
     if (n == 0) return;
     const QrTask* tasks = static_cast<const QrTask*>(ph.dTasks);
-    const int ns = ph.qrSmall;
-    if (ns > 0) {
-        if (ph.qrSmallK <= 24) qr_wave_kernel<T, 24><<<(ns + 3) / 4, 256, 0, s>>>(b, tasks, ns);
-        else qr_wave_kernel<T, 40><<<(ns + 3) / 4, 256, 0, s>>>(b, tasks, ns);
+    if (ph.qrClassEnd[3] > 0) {
+        if (ph.qrRegK <= 24) launch_qr_classes<T, 24>(tasks, ph.qrClassEnd, b, s);
+        else launch_qr_classes<T, 40>(tasks, ph.qrClassEnd, b, s);
     }
-    if (n > ns) qr_kernel<T><<<n - ns, 256, 0, s>>>(b, tasks + ns);
+    const int big = ph.qrClassEnd[3];
+    if (n > big) qr_kernel<T><<<n - big, 256, 0, s>>>(b, tasks + big);
     HIP_LAUNCH_CHECK();
 }
 
@@ -702,20 +774,22 @@ void launch_qr_phase(const Phase& ph, const Bases& b, hipStream_t s)
 // R = U S V^T; outputs G1 = V S (first `keep` columns, rest zero) and
 // G2 = U (same), both k x cap, and the kept rank.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256) svd_kernel(Bases b, const SvdTask* __restrict__ tasks,
-                                                  int tolMode, double relTol)
+// KMAX bounds k (LDS sized for it); NT threads, 8 per column pair.
+template <typename T, int KMAX, int NT>
+__global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restrict__ tasks,
+                                                 int tolMode, double relTol)
 {
-    __shared__ double As[64][65];
-    __shared__ double Vs[64][65];
-    __shared__ double sig[64];
-    __shared__ int order[64];
+    static_assert(NT >= 8 * (KMAX / 2), "one 8-thread group per column pair");
+    __shared__ double As[KMAX][KMAX + 1];
+    __shared__ double Vs[KMAX][KMAX + 1];
+    __shared__ double sig[KMAX];
+    __shared__ int order[KMAX];
     __shared__ int rotated;
     __shared__ int keep_s;
     const SvdTask tk = tasks[blockIdx.x];
     const int k = tk.k, tid = threadIdx.x;
     T* ws = reinterpret_cast<T*>(b.val[SLOT_WS]);
-    for (int e = tid; e < k * k; e += 256) {
+    for (int e = tid; e < k * k; e += NT) {
         const int r = e % k, c = e / k;
         As[c][r] = (double)ws[tk.roff + e];
         Vs[c][r] = (r == c) ? 1.0 : 0.0;
@@ -791,7 +865,7 @@ __global__ void __launch_bounds__(256) svd_kernel(Bases b, const SvdTask* __rest
     }
     __syncthreads();
     const int keep = keep_s;
-    for (int e = tid; e < k * tk.cap; e += 256) {
+    for (int e = tid; e < k * tk.cap; e += NT) {
         const int r = e % k, j = e / k;
         double g1 = 0.0, g2 = 0.0;
         if (j < keep) {
@@ -809,7 +883,11 @@ void launch_svd_phase(const Phase& ph, const Bases& b, TruncPolicy pol, hipStrea
 {
     const int n = (int)ph.svd.size();
     if (n == 0) return;
-    svd_kernel<T><<<n, 256, 0, s>>>(b, static_cast<const SvdTask*>(ph.dTasks), pol.tolMode, pol.relTol);
+    const SvdTask* t = static_cast<const SvdTask*>(ph.dTasks);
+    if (ph.svdK > 64) throw std::runtime_error("SVD factor wider than 64");
+    if (ph.svdK <= 24) svd_kernel<T, 24, 128><<<n, 128, 0, s>>>(b, t, pol.tolMode, pol.relTol);
+    else if (ph.svdK <= 40) svd_kernel<T, 40, 192><<<n, 192, 0, s>>>(b, t, pol.tolMode, pol.relTol);
+    else svd_kernel<T, 64, 256><<<n, 256, 0, s>>>(b, t, pol.tolMode, pol.relTol);
     HIP_LAUNCH_CHECK();
 }
 

@@ -311,49 +311,54 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     const int kS = maxRank + oversample;
     if (kS > 64) throw std::invalid_argument("maxRank + oversample must be <= 64");
     Plan* plan = new Plan();
+    // newPhase hands out references into plan->phases: capacity is reserved
+    // for every phase once the batches are known, so they stay valid.
+    auto newPhase = [&](PhaseKind k) -> Phase& {
+        if (plan->phases.size() == plan->phases.capacity())
+            throw std::logic_error("phase capacity exhausted");
+        plan->phases.emplace_back();
+        plan->phases.back().kind = k;
+        return plan->phases.back();
+    };
+    auto key2 = [](int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; };
 
-    // 1. Enumerate the product recursion and give every low-rank term its
-    //    factor panels.
+    // 1. Enumerate the product recursion; describe every low-rank term.
     ProductTerms terms;
     if (!add) terms = enumerateProductTerms(g);
     else {
         const size_t nb = g.blocks.size();
         terms.pushdowns.resize(nb); terms.lrTerms.resize(nb); terms.nnTerms.resize(nb); terms.ddTerms.resize(nb);
     }
-    std::vector<WFactor> W(terms.w.size());
-    struct Pending { int bA, bB; int wIndex; bool left; };
-    std::vector<Pending> wPending;
-    for (size_t wi = 0; wi < terms.w.size(); ++wi) {
-        const auto& tw = terms.w[wi];
-        const Block& A = g.blocks[tw.bA];
-        const Block& Bb = g.blocks[tw.bB];
-        const Block& C = g.blocks[tw.bC];
-        WFactor& w = W[wi];
-        w.level = C.level; w.t = C.t; w.s = C.s;
-        const Box& tb = g.boxes[C.level][C.t];
-        const Box& sb = g.boxes[C.level][C.s];
-        if (tw.leftLowRank) {
-            const int la = A.lrIndex;
-            w.cap = LA.cap[la];
-            w.PU = {SLOT_A, LA.uOff[la], tb.size};
-            w.PV = {SLOT_WS, wsAlloc((int64_t)sb.size * w.cap), sb.size};
-            w.rankIndex = la; w.rankSlot = SLOT_A;
-        } else {
-            const int lb = Bb.lrIndex;
-            w.cap = LB.cap[lb];
-            w.PU = {SLOT_WS, wsAlloc((int64_t)tb.size * w.cap), tb.size};
-            w.PV = {SLOT_B, LB.vOff[lb], sb.size};
-            w.rankIndex = lb; w.rankSlot = SLOT_B;
-        }
-        wPending.push_back({tw.bA, tw.bB, (int)wi, tw.leftLowRank});
-    }
-    const uint8_t abFlag = update ? CF_RUNTIME_ALPHA : 0;     // scaling of the A B terms in sketches
-    const uint8_t formFlag = (update || add) ? 0 : CF_RUNTIME_ALPHA;    // scaling applied when forming C.U
-    for (auto& w : W) w.cflag = abFlag;
     const auto& pushdowns = terms.pushdowns;
     const auto& lrTerms = terms.lrTerms;
     const auto& nnTerms = terms.nnTerms;
     const auto& ddTerms = terms.ddTerms;
+    const uint8_t abFlag = update ? CF_RUNTIME_ALPHA : 0;     // scaling of the A B terms in sketches
+    const uint8_t formFlag = (update || add) ? 0 : CF_RUNTIME_ALPHA;    // scaling applied when forming C.U
+    std::vector<WFactor> W(terms.w.size());
+    struct Pending { int bA, bB, bC; int wIndex; bool left; };
+    std::vector<Pending> wPending;
+    for (size_t wi = 0; wi < terms.w.size(); ++wi) {
+        const auto& tw = terms.w[wi];
+        const Block& C = g.blocks[tw.bC];
+        WFactor& w = W[wi];
+        w.level = C.level; w.t = C.t; w.s = C.s;
+        w.cflag = abFlag;
+        if (tw.leftLowRank) {
+            const int la = g.blocks[tw.bA].lrIndex;
+            w.cap = LA.cap[la];
+            w.PU = {SLOT_A, LA.uOff[la], g.boxes[C.level][C.t].size};
+            w.PV = {SLOT_WS, -1, g.boxes[C.level][C.s].size};      // allocated with its batch
+            w.rankIndex = la; w.rankSlot = SLOT_A;
+        } else {
+            const int lb = g.blocks[tw.bB].lrIndex;
+            w.cap = LB.cap[lb];
+            w.PU = {SLOT_WS, -1, g.boxes[C.level][C.t].size};
+            w.PV = {SLOT_B, LB.vOff[lb], g.boxes[C.level][C.s].size};
+            w.rankIndex = lb; w.rankSlot = SLOT_B;
+        }
+        wPending.push_back({tw.bA, tw.bB, tw.bC, (int)wi, tw.leftLowRank});
+    }
     // beta C_old for every admissible C block (update form); A and alpha B for
     // the blockwise sum (HMat2d::UpdateWith).
     std::vector<int> oldTerm(g.blocks.size(), -1), addTerm(g.blocks.size(), -1);
@@ -403,73 +408,122 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         if (addTerm[b] >= 0) { dst.push_back(addTerm[b]); dst.push_back(addTerm[b] + 1); }
     }
 
-    auto newPhase = [&](PhaseKind k) -> Phase& {
-        plan->phases.emplace_back();
-        plan->phases.back().kind = k;
-        return plan->phases.back();
-    };
-    plan->phases.reserve(16);
+    // 2. Batches. Every term lands in one C block. The C blocks are grouped
+    //    into units: the subtrees rooted at level `unitLevel`, and the leaves
+    //    and admissible blocks above that level. Batches of consecutive units
+    //    run one after the other and reuse one workspace region; only the
+    //    sketch matrices Omega, T1 = B Omega and the low-rank terms pushed
+    //    down from NODE blocks above unitLevel are shared by all batches.
+    //    unitLevel starts at 1 and deepens until every unit's estimated
+    //    workspace fits the budget.
+    const double budget = (double)wsBudget_;
+    std::vector<int> unit(g.blocks.size(), -1);
+    int nUnits = 0;
+    std::vector<double> est;
+    for (int unitLevel = 1;; ++unitLevel) {
+        std::fill(unit.begin(), unit.end(), -1);
+        nUnits = 0;
+        // DFS from the root: a block at unitLevel (or a leaf above it) opens a unit.
+        std::vector<std::pair<int, int>> stack{{g.root(), -1}};
+        while (!stack.empty()) {
+            const auto [b, u] = stack.back();
+            stack.pop_back();
+            const Block& B = g.blocks[b];
+            int ub = u;
+            if (ub < 0 && (B.level >= unitLevel || B.type != BT_NODE)) ub = nUnits++;
+            unit[b] = ub;
+            if (B.type == BT_NODE)
+                for (int c = g.nc() * g.nc() - 1; c >= 0; --c) stack.push_back({g.children[B.childBegin + c], ub});
+        }
+        // Workspace estimate per unit (elements), used only to group units.
+        est.assign(nUnits, 0.0);
+        for (int lbk : g.lrBlocks) {
+            const Block& C = g.blocks[lbk];
+            const int m = g.boxes[C.level][C.t].size, n = g.boxes[C.level][C.s].size;
+            double e = (double)(m + n) * kS + 3.0 * kS * kS;
+            if (C.level == g.numLevels - 1) e += (double)m * n;
+            e += 2.0 * kS * 64 * leafW[lbk].size();
+            for (const auto& tm : nnTerms[lbk]) {
+                const Block& A = g.blocks[tm.first];
+                e += (double)g.boxes[A.level][A.s].size * kS + 2.0 * 64 * kS * (A.leafEnd - A.leafBegin);
+            }
+            est[unit[lbk]] += e;
+        }
+        for (const auto& wp : wPending) {
+            if (unit[wp.bC] < 0) continue;
+            const WFactor& w = W[wp.wIndex];
+            const Block& Bl = g.blocks[wp.left ? wp.bB : wp.bA];
+            est[unit[wp.bC]] += (double)(wp.left ? w.PV.ld : w.PU.ld) * w.cap +
+                                64.0 * kS * (Bl.leafEnd - Bl.leafBegin);
+        }
+        const double worst = est.empty() ? 0.0 : *std::max_element(est.begin(), est.end());
+        if (worst <= budget || unitLevel >= g.numLevels - 1) break;
+    }
+    std::vector<std::vector<char>> groups;
+    {
+        double acc = 0;
+        for (int i = 0; i < nUnits; ++i) {
+            if (groups.empty() || (acc > 0 && acc + est[i] > budget)) {
+                groups.emplace_back(nUnits, 0);
+                acc = 0;
+            }
+            groups.back()[i] = 1;
+            acc += est[i];
+        }
+    }
+    auto inBatch = [&](const std::vector<char>& grp, int b) { return unit[b] >= 0 && grp[unit[b]]; };
 
-    // Sketch panels per admissible C block.
-    struct Sk { int b; int64_t omega, Y, Z, R, G1, G2, D; int m, n; bool leaf; };
-    std::vector<Sk> sk;
+    plan->phases.reserve(3 + 13 * groups.size());
+    // Exact factor of a low-rank term: the side that is not a stored factor
+    // of A or B is a sub-H product, written to a fresh workspace panel.
+    auto buildW = [&](PhaseBuilder& proj, PhaseBuilder& acc, const Pending& wp) {
+        WFactor& w = W[wp.wIndex];
+        const Block& A = g.blocks[wp.bA];
+        const Block& B = g.blocks[wp.bB];
+        if (wp.left) {
+            // PV (|s| x cap) = B(u,s)^T V_A, V_A rows = u box
+            w.PV.off = wsAlloc((int64_t)w.PV.ld * w.cap);
+            PanelTiles pt = addPanelTiles(acc, w.PV, w.level, w.s, w.cap, w.rankIndex, w.rankSlot, 0);
+            Operand in{SLOT_A, LA.vOff[A.lrIndex], g.boxes[A.level][A.s].size};
+            subh(proj, acc, SLOT_B, LB, wp.bB, true, in, A.level, A.s, pt, w.cap, w.rankIndex,
+                 w.rankSlot, 1.0f, 0, &plan->flops);
+        } else {
+            // PU (|t| x cap) = A(t,u) U_B, U_B rows = u box
+            w.PU.off = wsAlloc((int64_t)w.PU.ld * w.cap);
+            PanelTiles pt = addPanelTiles(acc, w.PU, w.level, w.t, w.cap, w.rankIndex, w.rankSlot, 0);
+            Operand in{SLOT_B, LB.uOff[B.lrIndex], g.boxes[B.level][B.t].size};
+            subh(proj, acc, SLOT_A, LA, wp.bA, false, in, B.level, B.t, pt, w.cap, w.rankIndex,
+                 w.rankSlot, 1.0f, 0, &plan->flops);
+        }
+    };
+
+    // 3. Shared sketch matrices Omega_s (per level and source box) and
+    //    T1 = B(u,s) Omega_s for the NODE*NODE terms of non-leaf blocks.
     std::unordered_map<uint64_t, int64_t> omegaOf;     // (level, s) -> panel
     std::unordered_map<uint64_t, int64_t> t1Of;        // B block -> T1 panel
-    auto key2 = [](int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; };
-    Phase& rngPh = newPhase(PH_RNG);
-    for (int lbk : g.lrBlocks) {
-        const Block& C = g.blocks[lbk];
-        const Box& tb = g.boxes[C.level][C.t];
-        const Box& sb = g.boxes[C.level][C.s];
-        Sk s;
-        s.b = lbk; s.m = tb.size; s.n = sb.size;
-        // Leaf-level admissible blocks are accumulated densely, then compressed.
-        s.leaf = (C.level == g.numLevels - 1);
-        s.D = s.leaf ? wsAlloc((int64_t)tb.size * sb.size) : -1;
-        auto it = omegaOf.find(key2(C.level, C.s));
-        if (it == omegaOf.end()) {
-            const int64_t o = wsAlloc((int64_t)sb.size * kS);
+    {
+        Phase& rngPh = newPhase(PH_RNG);
+        for (int lbk : g.lrBlocks) {
+            const Block& C = g.blocks[lbk];
+            if (omegaOf.count(key2(C.level, C.s))) continue;
+            const int n = g.boxes[C.level][C.s].size;
+            const int64_t o = wsAlloc((int64_t)n * kS);
             omegaOf[key2(C.level, C.s)] = o;
             RngTask r{};
-            r.off = o; r.m = sb.size; r.k = kS; r.stream = key2(C.level, C.s);
+            r.off = o; r.m = n; r.k = kS; r.stream = key2(C.level, C.s);
             rngPh.rng.push_back(r);
-            s.omega = o;
-        } else s.omega = it->second;
-        s.Y = wsAlloc((int64_t)tb.size * kS);
-        s.Z = wsAlloc((int64_t)sb.size * kS);
-        s.R = wsAlloc((int64_t)kS * kS);
-        s.G1 = wsAlloc((int64_t)kS * LC.cap[C.lrIndex]);
-        s.G2 = wsAlloc((int64_t)kS * LC.cap[C.lrIndex]);
-        sk.push_back(s);
-    }
-
-    // 2. W factors and T1 = B(u,s) Omega_s panels.
-    {
-        Phase& p1 = newPhase(PH_TILE);
-        p1.role = 1;
-        Phase& a1 = newPhase(PH_TILE);
-        PhaseBuilder proj(p1), acc(a1);
-        for (const auto& wp : wPending) {
-            const WFactor& w = W[wp.wIndex];
-            const Block& A = g.blocks[wp.bA];
-            const Block& B = g.blocks[wp.bB];
-            if (wp.left) {
-                // PV (|s| x cap) = B(u,s)^T V_A, V_A rows = u box
-                PanelTiles pt = addPanelTiles(acc, w.PV, w.level, w.s, w.cap, w.rankIndex, w.rankSlot, 0);
-                Operand in{SLOT_A, LA.vOff[A.lrIndex], g.boxes[A.level][A.s].size};
-                subh(proj, acc, SLOT_B, LB, wp.bB, true, in, A.level, A.s, pt, w.cap, w.rankIndex,
-                     w.rankSlot, 1.0f, 0, &plan->flops);
-            } else {
-                // PU (|t| x cap) = A(t,u) U_B, U_B rows = u box
-                PanelTiles pt = addPanelTiles(acc, w.PU, w.level, w.t, w.cap, w.rankIndex, w.rankSlot, 0);
-                Operand in{SLOT_B, LB.uOff[B.lrIndex], g.boxes[B.level][B.t].size};
-                subh(proj, acc, SLOT_A, LA, wp.bA, false, in, B.level, B.t, pt, w.cap, w.rankIndex,
-                     w.rankSlot, 1.0f, 0, &plan->flops);
-            }
         }
-        for (const auto& s : sk)
-            for (const auto& tm : nnTerms[s.b]) {
-                if (s.leaf || t1Of.count((uint64_t)tm.second)) continue;
+        Phase& p0 = newPhase(PH_TILE);
+        p0.role = 1;
+        Phase& a0 = newPhase(PH_TILE);
+        PhaseBuilder proj(p0), acc(a0);
+        for (const auto& wp : wPending)
+            if (unit[wp.bC] < 0) buildW(proj, acc, wp);
+        for (int lbk : g.lrBlocks) {
+            const Block& C = g.blocks[lbk];
+            if (C.level == g.numLevels - 1) continue;
+            for (const auto& tm : nnTerms[lbk]) {
+                if (t1Of.count((uint64_t)tm.second)) continue;
                 const Block& B = g.blocks[tm.second];
                 const Box& ub = g.boxes[B.level][B.t];
                 const int64_t T1 = wsAlloc((int64_t)ub.size * kS);
@@ -479,9 +533,13 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 subh(proj, acc, SLOT_B, LB, tm.second, false, in, B.level, B.s, pt, kS, -1, 0, 1.0f, 0,
                      &plan->flops);
             }
+        }
         proj.finalize();
         acc.finalize();
+        if (p0.tiles.empty() && a0.tiles.empty()) plan->phases.resize(plan->phases.size() - 2);
     }
+    const int64_t globalEnd = wsUsed();
+    int64_t wsPeak = globalEnd;
 
     // Low-rank term restricted to C block (t,s): PU rows of t, PV rows of s.
     auto slicePU = [&](const WFactor& w, const Box& tb) {
@@ -506,7 +564,6 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         }
         plan->flops += 2.0 * bx.size * w.cap * kS;
     };
-
     // Dense accumulation of a leaf-level C block: out (|t| x |s|) =
     // sum D_A D_B + sum PU PV^T over its dense*dense and low-rank terms.
     auto denseAccumulate = [&](PhaseBuilder& acc, int b, const Operand& out,
@@ -541,175 +598,219 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             plan->flops += 2.0 * tb.size * w.cap * sb.size;
         }
     };
-    {
-        Phase& ad = newPhase(PH_TILE);
-        PhaseBuilder acc(ad);
-        for (int bd : add ? std::vector<int>() : g.denseBlocks) {
-            const Block& C = g.blocks[bd];
-            // Dense C leaves: C = alpha (sum of terms) (+ beta C_old, read in place).
-            denseAccumulate(acc, bd, {SLOT_C, LC.denseOff[C.denseIndex], g.boxes[C.level][C.t].size},
-                            ddTerms[bd], CF_RUNTIME_ALPHA, update ? TF_RUNTIME_BETA : 0);
-        }
-        for (const auto& s : sk)
-            if (s.leaf) denseAccumulate(acc, s.b, {SLOT_WS, s.D, s.m}, nnTerms[s.b], abFlag, 0);
-        acc.finalize();
-    }
 
-    // 3. Y = T Omega for admissible C blocks.
-    {
-        Phase& p2 = newPhase(PH_TILE);
-        p2.role = 1;
-        Phase& a2 = newPhase(PH_TILE);
-        PhaseBuilder proj(p2), acc(a2);
-        for (size_t i = 0; i < sk.size(); ++i) {
-            const Sk& s = sk[i];
-            const Block& C = g.blocks[s.b];
+    // 4. Per batch: W factors, dense accumulation, Y = T Omega, QR, Z = T^T Q,
+    //    QR + SVD, formation of C.U and C.V.
+    struct Sk { int b; int64_t omega, Y, Z, R, G1, G2, D; int m, n; bool leaf; };
+    for (const auto& inGroup : groups) {
+        ws_ = globalEnd;
+        std::vector<Sk> sk;
+        for (int lbk : g.lrBlocks) {
+            if (!inBatch(inGroup, lbk)) continue;
+            const Block& C = g.blocks[lbk];
             const Box& tb = g.boxes[C.level][C.t];
             const Box& sb = g.boxes[C.level][C.s];
-            PanelTiles pt = addPanelTiles(acc, {SLOT_WS, s.Y, s.m}, C.level, C.t, kS, -1, 0, 0);
-            const Operand om{SLOT_WS, s.omega, s.n};
-            if (s.leaf) {
-                for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
-                    const int ra = g.atomStart[a] - tb.off;
-                    acc.add(pt.tileOfAtom[a - tb.atomBegin],
-                            mkc(SLOT_WS, s.D + ra, s.m, SLOT_WS, s.omega, s.n, s.n, -1, 0, 0, 1.0f));
+            Sk s;
+            s.b = lbk; s.m = tb.size; s.n = sb.size;
+            // Leaf-level admissible blocks are accumulated densely, then compressed.
+            s.leaf = (C.level == g.numLevels - 1);
+            s.D = s.leaf ? wsAlloc((int64_t)tb.size * sb.size) : -1;
+            s.omega = omegaOf[key2(C.level, C.s)];
+            s.Y = wsAlloc((int64_t)tb.size * kS);
+            s.Z = wsAlloc((int64_t)sb.size * kS);
+            s.R = wsAlloc((int64_t)kS * kS);
+            s.G1 = wsAlloc((int64_t)kS * LC.cap[C.lrIndex]);
+            s.G2 = wsAlloc((int64_t)kS * LC.cap[C.lrIndex]);
+            sk.push_back(s);
+        }
+        // W factors of this batch.
+        {
+            Phase& p1 = newPhase(PH_TILE);
+            p1.role = 1;
+            Phase& a1 = newPhase(PH_TILE);
+            PhaseBuilder proj(p1), acc(a1);
+            for (const auto& wp : wPending)
+                if (inBatch(inGroup, wp.bC)) buildW(proj, acc, wp);
+            proj.finalize();
+            acc.finalize();
+        }
+        {
+            Phase& ad = newPhase(PH_TILE);
+            PhaseBuilder acc(ad);
+            if (!add)
+                for (int bd : g.denseBlocks) {
+                    if (!inBatch(inGroup, bd)) continue;
+                    const Block& C = g.blocks[bd];
+                    // Dense C leaves: C = alpha (sum of terms) (+ beta C_old, read in place).
+                    denseAccumulate(acc, bd, {SLOT_C, LC.denseOff[C.denseIndex], g.boxes[C.level][C.t].size},
+                                    ddTerms[bd], CF_RUNTIME_ALPHA, update ? TF_RUNTIME_BETA : 0);
                 }
-                plan->flops += 2.0 * s.m * s.n * kS;
-                continue;
-            }
-            for (int wi : leafW[s.b]) {
-                const WFactor& w = W[wi];
-                const int64_t P = wsAlloc((int64_t)w.cap * kS);
-                projTile(proj, slicePV(w, sb), sb, om, w, P);
-                const Operand pu = slicePU(w, tb);
-                for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
-                    const int ra = g.atomStart[a] - tb.off;
-                    acc.add(pt.tileOfAtom[a - tb.atomBegin],
-                            mkc(pu.slot, pu.off + ra, pu.ld, SLOT_WS, P, w.cap, w.cap, w.rankIndex,
-                                w.rankSlot, w.cflag, 1.0f));
+            for (const auto& s : sk)
+                if (s.leaf) denseAccumulate(acc, s.b, {SLOT_WS, s.D, s.m}, nnTerms[s.b], abFlag, 0);
+            acc.finalize();
+        }
+        // Y = T Omega for admissible C blocks.
+        {
+            Phase& p2 = newPhase(PH_TILE);
+            p2.role = 1;
+            Phase& a2 = newPhase(PH_TILE);
+            PhaseBuilder proj(p2), acc(a2);
+            for (const Sk& s : sk) {
+                const Block& C = g.blocks[s.b];
+                const Box& tb = g.boxes[C.level][C.t];
+                const Box& sb = g.boxes[C.level][C.s];
+                PanelTiles pt = addPanelTiles(acc, {SLOT_WS, s.Y, s.m}, C.level, C.t, kS, -1, 0, 0);
+                const Operand om{SLOT_WS, s.omega, s.n};
+                if (s.leaf) {
+                    for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
+                        const int ra = g.atomStart[a] - tb.off;
+                        acc.add(pt.tileOfAtom[a - tb.atomBegin],
+                                mkc(SLOT_WS, s.D + ra, s.m, SLOT_WS, s.omega, s.n, s.n, -1, 0, 0, 1.0f));
+                    }
+                    plan->flops += 2.0 * s.m * s.n * kS;
+                    continue;
                 }
-                plan->flops += 2.0 * tb.size * w.cap * kS;
-            }
-            for (const auto& tm : nnTerms[s.b]) {
-                const Block& B = g.blocks[tm.second];
-                const Operand in{SLOT_WS, t1Of[(uint64_t)tm.second], g.boxes[B.level][B.t].size};
-                subh(proj, acc, SLOT_A, LA, tm.first, false, in, B.level, B.t, pt, kS, -1, 0, 1.0f, abFlag,
-                     &plan->flops);
-            }
-        }
-        proj.finalize();
-        acc.finalize();
-    }
-
-    // 4. Q = qr(Y)
-    {
-        Phase& q = newPhase(PH_QR);
-        q.qrK = kS;
-        for (const auto& s : sk) {
-            QrTask t{};
-            t.off = s.Y; t.roff = -1; t.m = s.m; t.k = kS; t.slot = SLOT_WS;
-            q.qr.push_back(t);
-            plan->flops += 4.0 * s.m * kS * kS;
-        }
-    }
-
-    // 5. Z = T^T Q: low-rank terms directly, NN terms via T2 = A(t,u)^T Q.
-    {
-        Phase& p3 = newPhase(PH_TILE);
-        p3.role = 1;
-        Phase& a3 = newPhase(PH_TILE);
-        Phase& p4 = newPhase(PH_TILE);
-        p4.role = 1;
-        Phase& a4 = newPhase(PH_TILE);
-        PhaseBuilder proj3(p3), acc3(a3), proj4(p4), acc4(a4);
-        for (size_t i = 0; i < sk.size(); ++i) {
-            const Sk& s = sk[i];
-            const Block& C = g.blocks[s.b];
-            const Box& tb = g.boxes[C.level][C.t];
-            const Box& sb = g.boxes[C.level][C.s];
-            PanelTiles zt = addPanelTiles(acc4, {SLOT_WS, s.Z, s.n}, C.level, C.s, kS, -1, 0, 0);
-            const Operand Q{SLOT_WS, s.Y, s.m};
-            if (s.leaf) {
-                // Z = D^T Q: L(i, p) = D(p, i)
-                for (int a = sb.atomBegin; a < sb.atomEnd; ++a) {
-                    const int ra = g.atomStart[a] - sb.off;
-                    acc4.add(zt.tileOfAtom[a - sb.atomBegin],
-                             mkc(SLOT_WS, s.D + (int64_t)ra * s.m, s.m, SLOT_WS, s.Y, s.m, s.m, -1, 0, CF_TRANS_L,
-                                 1.0f));
+                for (int wi : leafW[s.b]) {
+                    const WFactor& w = W[wi];
+                    const int64_t P = wsAlloc((int64_t)w.cap * kS);
+                    projTile(proj, slicePV(w, sb), sb, om, w, P);
+                    const Operand pu = slicePU(w, tb);
+                    for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
+                        const int ra = g.atomStart[a] - tb.off;
+                        acc.add(pt.tileOfAtom[a - tb.atomBegin],
+                                mkc(pu.slot, pu.off + ra, pu.ld, SLOT_WS, P, w.cap, w.cap, w.rankIndex,
+                                    w.rankSlot, w.cflag, 1.0f));
+                    }
+                    plan->flops += 2.0 * tb.size * w.cap * kS;
                 }
-                plan->flops += 2.0 * s.m * s.n * kS;
-                continue;
-            }
-            for (int wi : leafW[s.b]) {
-                const WFactor& w = W[wi];
-                const int64_t P = wsAlloc((int64_t)w.cap * kS);
-                projTile(proj3, slicePU(w, tb), tb, Q, w, P);
-                const Operand pv = slicePV(w, sb);
-                for (int a = sb.atomBegin; a < sb.atomEnd; ++a) {
-                    const int ra = g.atomStart[a] - sb.off;
-                    acc4.add(zt.tileOfAtom[a - sb.atomBegin],
-                             mkc(pv.slot, pv.off + ra, pv.ld, SLOT_WS, P, w.cap, w.cap, w.rankIndex,
-                                 w.rankSlot, w.cflag, 1.0f));
+                for (const auto& tm : nnTerms[s.b]) {
+                    const Block& B = g.blocks[tm.second];
+                    const Operand in{SLOT_WS, t1Of[(uint64_t)tm.second], g.boxes[B.level][B.t].size};
+                    subh(proj, acc, SLOT_A, LA, tm.first, false, in, B.level, B.t, pt, kS, -1, 0, 1.0f, abFlag,
+                         &plan->flops);
                 }
-                plan->flops += 2.0 * sb.size * w.cap * kS;
             }
-            for (const auto& tm : nnTerms[s.b]) {
-                const Block& A = g.blocks[tm.first];
-                const Box& ub = g.boxes[A.level][A.s];
-                const int64_t T2 = wsAlloc((int64_t)ub.size * kS);
-                PanelTiles pt = addPanelTiles(acc3, {SLOT_WS, T2, ub.size}, A.level, A.s, kS, -1, 0, 0);
-                subh(proj3, acc3, SLOT_A, LA, tm.first, true, Q, C.level, C.t, pt, kS, -1, 0, 1.0f, 0,
-                     &plan->flops);
-                subh(proj4, acc4, SLOT_B, LB, tm.second, true, {SLOT_WS, T2, ub.size}, A.level, A.s, zt, kS,
-                     -1, 0, 1.0f, abFlag, &plan->flops);
+            proj.finalize();
+            acc.finalize();
+        }
+        // Q = qr(Y)
+        {
+            Phase& q = newPhase(PH_QR);
+            q.qrK = kS;
+            for (const Sk& s : sk) {
+                QrTask t{};
+                t.off = s.Y; t.roff = -1; t.m = s.m; t.k = kS; t.slot = SLOT_WS;
+                q.qr.push_back(t);
+                plan->flops += 4.0 * s.m * kS * kS;
             }
         }
-        proj3.finalize(); acc3.finalize(); proj4.finalize(); acc4.finalize();
-    }
-
-    // 6. QR(Z) keeping R, 7. SVD(R) with rank decision, 8. form C.U, C.V.
-    {
-        Phase& q = newPhase(PH_QR);
-        q.qrK = kS;
-        Phase& sv = newPhase(PH_SVD);
-        for (const auto& s : sk) {
-            QrTask t{};
-            t.off = s.Z; t.roff = s.R; t.m = s.n; t.k = kS; t.slot = SLOT_WS;
-            q.qr.push_back(t);
-            plan->flops += 4.0 * s.n * kS * kS;
-            const Block& C = g.blocks[s.b];
-            SvdTask v{};
-            v.roff = s.R; v.g1off = s.G1; v.g2off = s.G2; v.k = kS;
-            v.cap = LC.cap[C.lrIndex];
-            v.rankIndex = C.lrIndex;
-            v.maxKeep = std::min(std::min(maxRank, v.cap), std::min(std::min(s.m, s.n), kS));
-            sv.svd.push_back(v);
-            plan->flops += 30.0 * kS * kS * kS;
+        // Z = T^T Q: low-rank terms directly, NN terms via T2 = A(t,u)^T Q.
+        {
+            Phase& p3 = newPhase(PH_TILE);
+            p3.role = 1;
+            Phase& a3 = newPhase(PH_TILE);
+            Phase& p4 = newPhase(PH_TILE);
+            p4.role = 1;
+            Phase& a4 = newPhase(PH_TILE);
+            PhaseBuilder proj3(p3), acc3(a3), proj4(p4), acc4(a4);
+            for (const Sk& s : sk) {
+                const Block& C = g.blocks[s.b];
+                const Box& tb = g.boxes[C.level][C.t];
+                const Box& sb = g.boxes[C.level][C.s];
+                PanelTiles zt = addPanelTiles(acc4, {SLOT_WS, s.Z, s.n}, C.level, C.s, kS, -1, 0, 0);
+                const Operand Q{SLOT_WS, s.Y, s.m};
+                if (s.leaf) {
+                    // Z = D^T Q: L(i, p) = D(p, i)
+                    for (int a = sb.atomBegin; a < sb.atomEnd; ++a) {
+                        const int ra = g.atomStart[a] - sb.off;
+                        acc4.add(zt.tileOfAtom[a - sb.atomBegin],
+                                 mkc(SLOT_WS, s.D + (int64_t)ra * s.m, s.m, SLOT_WS, s.Y, s.m, s.m, -1, 0,
+                                     CF_TRANS_L, 1.0f));
+                    }
+                    plan->flops += 2.0 * s.m * s.n * kS;
+                    continue;
+                }
+                for (int wi : leafW[s.b]) {
+                    const WFactor& w = W[wi];
+                    const int64_t P = wsAlloc((int64_t)w.cap * kS);
+                    projTile(proj3, slicePU(w, tb), tb, Q, w, P);
+                    const Operand pv = slicePV(w, sb);
+                    for (int a = sb.atomBegin; a < sb.atomEnd; ++a) {
+                        const int ra = g.atomStart[a] - sb.off;
+                        acc4.add(zt.tileOfAtom[a - sb.atomBegin],
+                                 mkc(pv.slot, pv.off + ra, pv.ld, SLOT_WS, P, w.cap, w.cap, w.rankIndex,
+                                     w.rankSlot, w.cflag, 1.0f));
+                    }
+                    plan->flops += 2.0 * sb.size * w.cap * kS;
+                }
+                for (const auto& tm : nnTerms[s.b]) {
+                    const Block& A = g.blocks[tm.first];
+                    const Box& ub = g.boxes[A.level][A.s];
+                    const int64_t T2 = wsAlloc((int64_t)ub.size * kS);
+                    PanelTiles pt = addPanelTiles(acc3, {SLOT_WS, T2, ub.size}, A.level, A.s, kS, -1, 0, 0);
+                    subh(proj3, acc3, SLOT_A, LA, tm.first, true, Q, C.level, C.t, pt, kS, -1, 0, 1.0f, 0,
+                         &plan->flops);
+                    subh(proj4, acc4, SLOT_B, LB, tm.second, true, {SLOT_WS, T2, ub.size}, A.level, A.s, zt, kS,
+                         -1, 0, 1.0f, abFlag, &plan->flops);
+                }
+            }
+            proj3.finalize(); acc3.finalize(); proj4.finalize(); acc4.finalize();
         }
-        Phase& f = newPhase(PH_TILE);
-        PhaseBuilder acc(f);
-        for (const auto& s : sk) {
-            const Block& C = g.blocks[s.b];
-            const int cap = LC.cap[C.lrIndex];
-            const Box& tb = g.boxes[C.level][C.t];
-            const Box& sb = g.boxes[C.level][C.s];
-            PanelTiles ut = addPanelTiles(acc, {SLOT_C, LC.uOff[C.lrIndex], s.m}, C.level, C.t, cap, -1, 0, 0);
-            PanelTiles vt = addPanelTiles(acc, {SLOT_C, LC.vOff[C.lrIndex], s.n}, C.level, C.s, cap, -1, 0, 0);
-            for (int a = tb.atomBegin; a < tb.atomEnd; ++a)
-                for (int c = 0; c < ut.kChunks; ++c)
-                    acc.add(ut.tileOfAtom[(size_t)(a - tb.atomBegin) * ut.kChunks + c],
-                            mkc(SLOT_WS, s.Y + (g.atomStart[a] - tb.off), s.m, SLOT_WS,
-                                s.G1 + (int64_t)c * 64 * kS, kS, kS, -1, 0, formFlag, 1.0f));
-            for (int a = sb.atomBegin; a < sb.atomEnd; ++a)
-                for (int c = 0; c < vt.kChunks; ++c)
-                    acc.add(vt.tileOfAtom[(size_t)(a - sb.atomBegin) * vt.kChunks + c],
-                            mkc(SLOT_WS, s.Z + (g.atomStart[a] - sb.off), s.n, SLOT_WS,
-                                s.G2 + (int64_t)c * 64 * kS, kS, kS, -1, 0, 0, 1.0f));
-            plan->flops += 2.0 * (s.m + s.n) * kS * cap;
+        // QR(Z) keeping R, SVD(R) with rank decision, form C.U, C.V.
+        {
+            Phase& q = newPhase(PH_QR);
+            q.qrK = kS;
+            Phase& sv = newPhase(PH_SVD);
+            for (const Sk& s : sk) {
+                QrTask t{};
+                t.off = s.Z; t.roff = s.R; t.m = s.n; t.k = kS; t.slot = SLOT_WS;
+                q.qr.push_back(t);
+                plan->flops += 4.0 * s.n * kS * kS;
+                const Block& C = g.blocks[s.b];
+                SvdTask v{};
+                v.roff = s.R; v.g1off = s.G1; v.g2off = s.G2; v.k = kS;
+                v.cap = LC.cap[C.lrIndex];
+                v.rankIndex = C.lrIndex;
+                v.maxKeep = std::min(std::min(maxRank, v.cap), std::min(std::min(s.m, s.n), kS));
+                sv.svd.push_back(v);
+                plan->flops += 30.0 * kS * kS * kS;
+            }
+            Phase& f = newPhase(PH_TILE);
+            PhaseBuilder acc(f);
+            for (const Sk& s : sk) {
+                const Block& C = g.blocks[s.b];
+                const int cap = LC.cap[C.lrIndex];
+                const Box& tb = g.boxes[C.level][C.t];
+                const Box& sb = g.boxes[C.level][C.s];
+                PanelTiles ut = addPanelTiles(acc, {SLOT_C, LC.uOff[C.lrIndex], s.m}, C.level, C.t, cap, -1, 0, 0);
+                PanelTiles vt = addPanelTiles(acc, {SLOT_C, LC.vOff[C.lrIndex], s.n}, C.level, C.s, cap, -1, 0, 0);
+                for (int a = tb.atomBegin; a < tb.atomEnd; ++a)
+                    for (int c = 0; c < ut.kChunks; ++c)
+                        acc.add(ut.tileOfAtom[(size_t)(a - tb.atomBegin) * ut.kChunks + c],
+                                mkc(SLOT_WS, s.Y + (g.atomStart[a] - tb.off), s.m, SLOT_WS,
+                                    s.G1 + (int64_t)c * 64 * kS, kS, kS, -1, 0, formFlag, 1.0f));
+                for (int a = sb.atomBegin; a < sb.atomEnd; ++a)
+                    for (int c = 0; c < vt.kChunks; ++c)
+                        acc.add(vt.tileOfAtom[(size_t)(a - sb.atomBegin) * vt.kChunks + c],
+                                mkc(SLOT_WS, s.Z + (g.atomStart[a] - sb.off), s.n, SLOT_WS,
+                                    s.G2 + (int64_t)c * 64 * kS, kS, kS, -1, 0, 0, 1.0f));
+                plan->flops += 2.0 * (s.m + s.n) * kS * cap;
+            }
+            acc.finalize();
         }
-        acc.finalize();
+        wsPeak = std::max(wsPeak, wsUsed());
     }
-    plan->wsElems = wsUsed();
+    // Drop phases with nothing to do (small geometries, empty batches).
+    std::vector<Phase> kept;
+    kept.reserve(plan->phases.size());
+    for (auto& ph : plan->phases) {
+        const bool empty = ph.tiles.empty() && ph.qr.empty() && ph.svd.empty() && ph.rng.empty();
+        if (!empty) kept.push_back(std::move(ph));
+    }
+    plan->phases.swap(kept);
+    plan->wsElems = wsPeak;
+    plan->wsShared = globalEnd;
+    plan->batches = (int)groups.size();
     for (auto& ph : plan->phases) {
         plan->tileCount += ph.tiles.size();
         plan->contribCount += ph.contribs.size();

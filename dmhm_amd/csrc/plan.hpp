@@ -115,8 +115,9 @@ struct Phase {
     int mvK = 0, ldx = 0, ldy = 0;
     void* dAux = nullptr;
     int qrK = 0;            // panel width of the QR phase
-    int qrSmall = 0;        // tasks [0, qrSmall) fit one wave (m <= 64, k <= 40), set by upload
-    int qrSmallK = 0;       // widest of those tasks
+    int qrClassEnd[4] = {0, 0, 0, 0};   // QR tasks sorted by panel class at upload (see launch_qr_phase)
+    int qrRegK = 0;         // widest task of classes 0..3
+    int svdK = 0;           // largest SVD factor of the phase (set by upload)
     // device copies (filled by upload)
     void* dTiles = nullptr; void* dContribs = nullptr; void* dTasks = nullptr;
     int maxKTile = 0;       // widest tile (selects the kernel variant)
@@ -130,6 +131,8 @@ struct Plan {
     int64_t wsElems = 0;    // workspace elements (of the run-time precision)
     double flops = 0;       // algorithmic flops (GEMM shapes)
     int64_t contribCount = 0, tileCount = 0;
+    int batches = 1;        // H*H: number of level-1 subtree batches sharing the workspace
+    int64_t wsShared = 0;   // H*H: workspace elements shared by all batches (Omega, T1)
     void upload();          // copy tile/contrib/task arrays to the device
     void release();
     ~Plan() { release(); }
@@ -204,6 +207,9 @@ public:
         return o;
     }
     int64_t wsUsed() const { return ws_; }
+    // Workspace budget (elements) per H*H batch; the product is split into
+    // batches of level-1 subtrees whose estimated workspace fits it.
+    void setWorkspaceBudget(int64_t elems) { wsBudget_ = elems; }
 
     // Sub-H product: out(|outBox| x k) (+)= op(M(b)) in(|inBox| x k). Leaves of
     // M inside block b contribute to the tiles of `out` (registered per atom in
@@ -224,6 +230,7 @@ public:
 private:
     const Geometry& g_;
     int64_t ws_ = 0;
+    int64_t wsBudget_ = int64_t(1) << 31;
 };
 
 } // namespace dmhm
