@@ -377,8 +377,8 @@ static void rounded_update_dev(int maxRank, double alpha, int m, int n, int rA, 
     job.plan.phases.resize(5);
     Phase& q = job.plan.phases[0];
     q.kind = PH_QR;
-    q.qr.push_back(QrTask{Uc, R1, mp, r, SLOT_WS, 0, 0, 0, 0});
-    q.qr.push_back(QrTask{Vc, R2, np, r, SLOT_WS, 0, 0, 0, 0});
+    q.qr.push_back(QrTask{Uc, R1, mp, r, mp, r, SLOT_WS, 0, 0, 0, 0});
+    q.qr.push_back(QrTask{Vc, R2, np, r, np, r, SLOT_WS, 0, 0, 0, 0});
     job.plan.phases[1].kind = PH_TILE;                        // W = R1 R2^T
     add_gemm_tiles(job.plan.phases[1], Wm, r, r, R1, r, r, R2, r, CF_TRANS_R);
     Phase& sv = job.plan.phases[2];
@@ -425,7 +425,7 @@ static void compress_dense_dev(int maxRank, int m, int n, const double* D, doubl
     HIPCHECK(hipMalloc((void**)&job.rank, sizeof(int)));
     job.plan.phases.resize(3);
     job.plan.phases[0].kind = PH_QR;
-    job.plan.phases[0].qr.push_back(QrTask{Dd, R, mp, k, SLOT_WS, 0, 0, 0, 0});
+    job.plan.phases[0].qr.push_back(QrTask{Dd, R, mp, k, mp, k, SLOT_WS, 0, 0, 0, 0});
     job.plan.phases[1].kind = PH_SVD;                         // R = U S V^T: G1 = V S, G2 = U
     job.plan.phases[1].svd.push_back(SvdTask{R, G1, G2, k, k, 0, cap});
     job.plan.phases[2].kind = PH_TILE;                        // out U = Q U_R
@@ -1473,7 +1473,9 @@ void Plan::upload()
                 if (t.m <= 64) return 0;
                 if (t.m <= 256) return 1;
                 if (t.m <= 512) return 2;
-                return (t.m <= 1024 && t.k <= 24) ? 3 : 4;
+                if (t.m <= 1024 && t.k <= 24) return 3;
+                if (t.ld != t.m || t.rld != t.k) throw Error(DMHM_EINVAL, "strided QR panel outside the register classes");
+                return 4;
             };
             std::stable_sort(ph.qr.begin(), ph.qr.end(),
                              [&](const QrTask& x, const QrTask& y) { return cls(x) < cls(y); });

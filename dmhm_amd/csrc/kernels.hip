@@ -578,7 +578,8 @@ __global__ void __launch_bounds__(256) qr_kernel(Bases b, const QrTask* __restri
 // reduced together (wave sums, then the NW wave partials through LDS), and
 // the pivot row is published through LDS in the same round, so a step costs
 // one barrier. Q is formed in place (orgqr order). FP64 arithmetic; same
-// reflector convention and outputs as qr_kernel.
+// reflector convention and outputs as qr_kernel; the panel and R may be
+// strided (leading dimensions ld, rld), as the row blocks of a TSQR are.
 template <typename T, int KMAX, int NW, int RPL>
 __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* __restrict__ tasks)
 {
@@ -588,14 +589,14 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
     __shared__ double taus[KMAX];
     const QrTask tk = tasks[blockIdx.x];
     T* A = reinterpret_cast<T*>(b.val[tk.slot]) + tk.off;
-    const int m = tk.m, k = tk.k, kk = min(m, k);
+    const int m = tk.m, k = tk.k, kk = min(m, k), ld = tk.ld;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     double a[RPL][KMAX];
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
         const int i = tid + NT * r;
 #pragma unroll
-        for (int c = 0; c < KMAX; ++c) a[r][c] = (i < m && c < k) ? (double)A[i + (int64_t)c * m] : 0.0;
+        for (int c = 0; c < KMAX; ++c) a[r][c] = (i < m && c < k) ? (double)A[i + (int64_t)c * ld] : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
@@ -668,7 +669,7 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
             if (i < k) {
 #pragma unroll
                 for (int c = 0; c < KMAX; ++c)
-                    if (c < k) R[i + c * k] = (i <= c && i < kk) ? (T)a[r][c] : T(0);
+                    if (c < k) R[i + (int64_t)c * tk.rld] = (i <= c && i < kk) ? (T)a[r][c] : T(0);
             }
         }
     }
@@ -729,7 +730,7 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
         if (i < m) {
 #pragma unroll
             for (int c = 0; c < KMAX; ++c)
-                if (c < k) A[i + (int64_t)c * m] = (T)a[r][c];
+                if (c < k) A[i + (int64_t)c * ld] = (T)a[r][c];
         }
     }
 }

@@ -473,7 +473,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     }
     auto inBatch = [&](const std::vector<char>& grp, int b) { return unit[b] >= 0 && grp[unit[b]]; };
 
-    plan->phases.reserve(3 + 13 * groups.size());
+    plan->phases.reserve(3 + 17 * groups.size());
     // Exact factor of a low-rank term: the side that is not a stored factor
     // of A or B is a sub-H product, written to a fresh workspace panel.
     auto buildW = [&](PhaseBuilder& proj, PhaseBuilder& acc, const Pending& wp) {
@@ -599,6 +599,52 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         }
     };
 
+    // Householder QR of an m x kS panel of box (level, box) at `off` (Q in
+    // place, R to roff unless -1). Panels taller than one register-resident
+    // workgroup are factored as a TSQR: QR of row blocks of whole atoms
+    // (their R factors stacked), QR of the stack, then Q_i := Q_i Q_stack,i.
+    const int tsqrRows = tsqrRows_ > 0 ? tsqrRows_ : (kS <= 24 ? 1024 : 512);
+    auto addQr = [&](Phase& q1, Phase& q2, PhaseBuilder& comb, int64_t off, int level, int box, int64_t roff) {
+        const Box& B = g.boxes[level][box];
+        const int m = B.size;
+        plan->flops += 4.0 * m * kS * kS;
+        if (m <= tsqrRows || kS > 40) {
+            QrTask t{};
+            t.off = off; t.roff = roff; t.m = m; t.k = kS; t.ld = m; t.rld = kS; t.slot = SLOT_WS;
+            q1.qr.push_back(t);
+            return;
+        }
+        std::vector<int> blockAtom{B.atomBegin};
+        for (int a = B.atomBegin, rows = 0; a < B.atomEnd; ++a) {
+            if (rows + g.atomSize[a] > tsqrRows) { blockAtom.push_back(a); rows = 0; }
+            rows += g.atomSize[a];
+        }
+        blockAtom.push_back(B.atomEnd);
+        const int nb = (int)blockAtom.size() - 1;
+        const int64_t Rs = wsAlloc((int64_t)nb * kS * kS);
+        for (int i = 0; i < nb; ++i) {
+            const int r0 = g.atomStart[blockAtom[i]] - B.off;
+            const int r1 = g.atomStart[blockAtom[i + 1] - 1] + g.atomSize[blockAtom[i + 1] - 1] - B.off;
+            QrTask t{};
+            t.off = off + r0; t.roff = Rs + (int64_t)i * kS; t.m = r1 - r0; t.k = kS;
+            t.ld = m; t.rld = nb * kS; t.slot = SLOT_WS;
+            q1.qr.push_back(t);
+        }
+        QrTask t{};
+        t.off = Rs; t.roff = roff; t.m = nb * kS; t.k = kS; t.ld = nb * kS; t.rld = kS; t.slot = SLOT_WS;
+        q2.qr.push_back(t);
+        for (int i = 0; i < nb; ++i)
+            for (int a = blockAtom[i]; a < blockAtom[i + 1]; ++a) {
+                const int ra = g.atomStart[a] - B.off;
+                Tile tl{};
+                tl.off = off + ra; tl.ld = m; tl.m = g.atomSize[a]; tl.k = kS;
+                tl.mrank = tl.krank = -1; tl.slot = SLOT_WS;
+                const int id = comb.addTile(tl);
+                comb.add(id, mkc(SLOT_WS, off + ra, m, SLOT_WS, Rs + (int64_t)i * kS, nb * kS, kS, -1, 0, 0, 1.0f));
+                plan->flops += 2.0 * g.atomSize[a] * kS * kS;
+            }
+    };
+
     // 4. Per batch: W factors, dense accumulation, Y = T Omega, QR, Z = T^T Q,
     //    QR + SVD, formation of C.U and C.V.
     struct Sk { int b; int64_t omega, Y, Z, R, G1, G2, D; int m, n; bool leaf; };
@@ -696,13 +742,15 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         // Q = qr(Y)
         {
             Phase& q = newPhase(PH_QR);
-            q.qrK = kS;
+            Phase& q2 = newPhase(PH_QR);
+            Phase& cb = newPhase(PH_TILE);
+            q.qrK = q2.qrK = kS;
+            PhaseBuilder comb(cb);
             for (const Sk& s : sk) {
-                QrTask t{};
-                t.off = s.Y; t.roff = -1; t.m = s.m; t.k = kS; t.slot = SLOT_WS;
-                q.qr.push_back(t);
-                plan->flops += 4.0 * s.m * kS * kS;
+                const Block& C = g.blocks[s.b];
+                addQr(q, q2, comb, s.Y, C.level, C.t, -1);
             }
+            comb.finalize();
         }
         // Z = T^T Q: low-rank terms directly, NN terms via T2 = A(t,u)^T Q.
         {
@@ -759,13 +807,19 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         // QR(Z) keeping R, SVD(R) with rank decision, form C.U, C.V.
         {
             Phase& q = newPhase(PH_QR);
-            q.qrK = kS;
+            Phase& q2 = newPhase(PH_QR);
+            Phase& cb = newPhase(PH_TILE);
+            q.qrK = q2.qrK = kS;
+            {
+                PhaseBuilder comb(cb);
+                for (const Sk& s : sk) {
+                    const Block& C = g.blocks[s.b];
+                    addQr(q, q2, comb, s.Z, C.level, C.s, s.R);
+                }
+                comb.finalize();
+            }
             Phase& sv = newPhase(PH_SVD);
             for (const Sk& s : sk) {
-                QrTask t{};
-                t.off = s.Z; t.roff = s.R; t.m = s.n; t.k = kS; t.slot = SLOT_WS;
-                q.qr.push_back(t);
-                plan->flops += 4.0 * s.n * kS * kS;
                 const Block& C = g.blocks[s.b];
                 SvdTask v{};
                 v.roff = s.R; v.g1off = s.G1; v.g2off = s.G2; v.k = kS;

@@ -51,10 +51,11 @@ struct Tile {
     int32_t pad;
 };
 
-struct QrTask {              // Householder QR of an m x k panel (ld = m), Q in place
+struct QrTask {              // Householder QR of an m x k panel, Q in place
     int64_t off;            // panel offset (slot)
-    int64_t roff;           // R output (k x k, ld k) offset in the workspace, or -1
+    int64_t roff;           // R output (k x k upper triangle, ld rld) offset in the workspace, or -1
     int32_t m, k;
+    int32_t ld, rld;        // panel and R leading dimensions
     uint8_t slot, pad0, pad1, pad2;
     int32_t pad;
 };
@@ -210,6 +211,9 @@ public:
     // Workspace budget (elements) per H*H batch; the product is split into
     // batches of level-1 subtrees whose estimated workspace fits it.
     void setWorkspaceBudget(int64_t elems) { wsBudget_ = elems; }
+    // Row-block height of the TSQR of tall sketch panels (0: 1024 for sketch
+    // widths <= 24, else 512, the register-resident QR limits).
+    void setTsqrRows(int rows) { tsqrRows_ = rows; }
 
     // Sub-H product: out(|outBox| x k) (+)= op(M(b)) in(|inBox| x k). Leaves of
     // M inside block b contribute to the tiles of `out` (registered per atom in
@@ -231,6 +235,7 @@ private:
     const Geometry& g_;
     int64_t ws_ = 0;
     int64_t wsBudget_ = int64_t(1) << 31;
+    int tsqrRows_ = 0;
 };
 
 } // namespace dmhm
