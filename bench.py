@@ -147,13 +147,15 @@ def main():
                           "phases": X.last_hh_phases()}))
         return
 
-    # Warm-up: plan building, workspace, one iteration in each precision.
+    # Warm-up: plan building and workspace. Two iterations in each precision
+    # reach every plan of the loop (first product with the imported operator,
+    # then products of full-rank iterates).
     for w in range(a.warmup):
         log("bench: warmup %d" % w)
         X = A.Copy()
-        X.SchulzInvert(numIterations=1)
+        X.SchulzInvert(numIterations=2)
         X = A.Copy()
-        X.SchulzInvert(numIterations=1, mixed=True)
+        X.SchulzInvert(numIterations=2, mixed=True)
     dmhm_amd.load().dmhm_synchronize()
 
     log("bench: timing %d mixed Schulz inversion(s)" % a.steps)

@@ -127,6 +127,10 @@ struct dmhm_hmat_s {
     int maxRank = 0;
     void* vals = nullptr;
     int* ranks = nullptr;
+    // Admissible blocks expected at full capacity rank (products, iterates,
+    // SetToRandom); false for imports and zero matrices, whose rank-0 blocks
+    // the product plans then skip at run time instead of stacking them.
+    bool full = false;
     size_t esize() const { return prec == DMHM_F64 ? 8 : 4; }
     ~dmhm_hmat_s() {
         if (vals) (void)hipFree(vals);
@@ -172,6 +176,7 @@ static void run_plan(Plan& p, const Bases& b, T alpha, T beta, TruncPolicy pol, 
         case PH_QR: launch_qr_phase<T>(ph, b, s); break;
         case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
         case PH_RNG: launch_rng_phase<T>(ph, b, seed, s); break;
+        case PH_EYE: launch_eye<T>(reinterpret_cast<T*>(b.val[SLOT_WS]) + ph.zeroOff, (int)ph.zeroCount, s); break;
         case PH_MV_ZERO: case PH_MV_PROJ: case PH_MV_ACC: launch_mv_phase<T>(ph, b, alpha, beta, s); break;
         default: break;
         }
@@ -226,9 +231,10 @@ static dmhm_hmat_s* multiply_hmat(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, 
     const int over = o.oversample > 0 ? o.oversample : 4;
     const std::string key = "hh:" + std::to_string(A->lay->id) + ":" + std::to_string(B->lay->id) + ":" +
                             std::to_string(LC->id) + ":" + std::to_string(maxRank) + ":" +
-                            std::to_string(over);
+                            std::to_string(over) + ":" + std::to_string(A->full) + std::to_string(B->full);
     Plan& p = get_plan(gc, key, [&] {
         Planner pl(gc.geo);
+        pl.setStacking(A->full, B->full);
         return pl.hhProduct(*A->lay, *B->lay, *LC, maxRank, over);
     });
     ensure_ws(p, A->esize());
@@ -236,6 +242,7 @@ static dmhm_hmat_s* multiply_hmat(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, 
     TruncPolicy pol{o.tolMode, o.relTol};
     if (A->prec == DMHM_F64) run_plan<double>(p, b, alpha, 0.0, pol, o.seed);
     else run_plan<float>(p, b, (float)alpha, 0.0f, pol, o.seed);
+    C->full = true;
     return C.release();
 }
 
@@ -253,9 +260,11 @@ static void multiply_hmat_update(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, d
     std::unique_ptr<dmhm_hmat_s> Cold(copy_hmat(C, C->prec));
     const int over = o.oversample > 0 ? o.oversample : 4;
     const std::string key = "hhu:" + std::to_string(A->lay->id) + ":" + std::to_string(B->lay->id) + ":" +
-                            std::to_string(C->lay->id) + ":" + std::to_string(maxRank) + ":" + std::to_string(over);
+                            std::to_string(C->lay->id) + ":" + std::to_string(maxRank) + ":" + std::to_string(over) +
+                            ":" + std::to_string(A->full) + std::to_string(B->full);
     Plan& p = get_plan(gc, key, [&] {
         Planner pl(gc.geo);
+        pl.setStacking(A->full, B->full);
         return pl.hhProduct(*A->lay, *B->lay, *C->lay, maxRank, over, Planner::HH_UPDATE);
     });
     ensure_ws(p, A->esize());
@@ -265,6 +274,7 @@ static void multiply_hmat_update(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, d
     if (A->prec == DMHM_F64) run_plan<double>(p, b, alpha, beta, pol, o.seed);
     else run_plan<float>(p, b, (float)alpha, (float)beta, pol, o.seed);
     HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    C->full = true;
 }
 
 static void update_with(dmhm_hmat_s* A, double alpha, dmhm_hmat_s* B, const dmhm_hh_opts& o)
@@ -291,6 +301,7 @@ static void update_with(dmhm_hmat_s* A, double alpha, dmhm_hmat_s* B, const dmhm
         launch_axpby<float>((const float*)A->vals, (const float*)B->vals, (float)alpha, (float*)C->vals, nd, g_lib.stream);
     }
     HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    A->full = true;
     std::swap(A->lay, C->lay);
     std::swap(A->vals, C->vals);
     std::swap(A->ranks, C->ranks);
@@ -482,6 +493,7 @@ static dmhm_hmat_s* create_from_lowrank(int dim, const int* sizes, int L, int ma
     }
     std::vector<int> ranks(g.lrBlocks.size(), r);
     if (!ranks.empty()) HIPCHECK(hipMemcpy(h->ranks, ranks.data(), ranks.size() * sizeof(int), hipMemcpyHostToDevice));
+    h->full = r >= std::max(maxRank, 1);
     return h.release();
 }
 
@@ -494,6 +506,7 @@ static dmhm_hmat_s* copy_hmat(dmhm_hmat_s* s, int prec)
     else launch_convert<float, double>((const float*)s->vals, (double*)d->vals, n, g_lib.stream);
     HIPCHECK(hipMemcpyAsync(d->ranks, s->ranks, s->lay->nLowRank() * sizeof(int), hipMemcpyDeviceToDevice,
                             g_lib.stream));
+    d->full = s->full;
     return d.release();
 }
 
@@ -522,6 +535,7 @@ static dmhm_hmat_s* transpose_hmat(dmhm_hmat_s* s)
         Ld = std::make_shared<Layout>(Layout::make(g, caps, g_lib.nextLayoutId++));
     }
     std::unique_ptr<dmhm_hmat_s> d(alloc_hmat(s->gc, Ld, s->prec, s->maxRank));
+    d->full = s->full;
     std::vector<int64_t> so, dof; std::vector<int> mm, nn;
     for (size_t i = 0; i < g.denseBlocks.size(); ++i) {
         const int b = g.denseBlocks[i];
@@ -566,6 +580,7 @@ static void scale_hmat(dmhm_hmat_s* h, double alpha)
     const Layout& L = *h->lay;
     hipStream_t st = g_lib.stream;
     if (alpha == 0.0) {
+        h->full = false;
         // hmat_tools::Scale(0, LowRank) drops the rank (hmat_tools.hpp:1644-1648).
         HIPCHECK(hipMemsetAsync(h->vals, 0, L.totalElems() * h->esize(), st));
         launch_fill_int(h->ranks, L.nLowRank(), 0, st);
@@ -600,6 +615,7 @@ static void add_diag(dmhm_hmat_s* h, double alpha)
 
 static void set_to_random(dmhm_hmat_s* h, uint64_t seed)
 {
+    h->full = true;
     const Layout& L = *h->lay;
     hipStream_t st = g_lib.stream;
     // Dense leaves, then U and V at rank = maxRank (SetToRandom-incl.hpp:34-48);
@@ -726,6 +742,7 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
     if (low) { X.reset(copy_hmat(X.get(), DMHM_F64)); }
     HIPCHECK(hipStreamSynchronize(g_lib.stream));
     // Replace A's contents with X (CopyFrom, Invert-incl.hpp:195).
+    A->full = X->full;
     std::swap(A->lay, X->lay);
     std::swap(A->vals, X->vals);
     std::swap(A->ranks, X->ranks);
@@ -883,6 +900,7 @@ static void run_phase(Phase& ph, const Bases& b, T alpha, T beta, TruncPolicy po
     case PH_QR: launch_qr_phase<T>(ph, b, s); break;
     case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
     case PH_RNG: launch_rng_phase<T>(ph, b, seed, s); break;
+    case PH_EYE: launch_eye<T>(reinterpret_cast<T*>(b.val[SLOT_WS]) + ph.zeroOff, (int)ph.zeroCount, s); break;
     case PH_MV_ZERO: case PH_MV_PROJ: case PH_MV_ACC: launch_mv_phase<T>(ph, b, alpha, beta, s); break;
     default: break;
     }
@@ -965,8 +983,13 @@ static void profile_hh(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, const dmhm_
     std::unique_ptr<dmhm_hmat_s> C(alloc_hmat(A->gc, LC, A->prec, maxRank));
     const int over = o.oversample > 0 ? o.oversample : 4;
     const std::string key = "hh:" + std::to_string(A->lay->id) + ":" + std::to_string(B->lay->id) + ":" +
-                            std::to_string(LC->id) + ":" + std::to_string(maxRank) + ":" + std::to_string(over);
-    Plan& p = get_plan(gc, key, [&] { Planner pl(gc.geo); return pl.hhProduct(*A->lay, *B->lay, *LC, maxRank, over); });
+                            std::to_string(LC->id) + ":" + std::to_string(maxRank) + ":" + std::to_string(over) +
+                            ":" + std::to_string(A->full) + std::to_string(B->full);
+    Plan& p = get_plan(gc, key, [&] {
+        Planner pl(gc.geo);
+        pl.setStacking(A->full, B->full);
+        return pl.hhProduct(*A->lay, *B->lay, *LC, maxRank, over);
+    });
     ensure_ws(p, A->esize());
     Bases b = bases_for(A, B, C.get(), nullptr, nullptr);
     TruncPolicy pol{o.tolMode, o.relTol};

@@ -981,6 +981,20 @@ __global__ void fill_kernel(T* x, int64_t n, T v)
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
         x[e] = v;
 }
+template <typename T>
+__global__ void eye_kernel(T* x, int n)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n * n) x[e] = (e % n == e / n) ? T(1) : T(0);
+}
+
+template <typename T> void launch_eye(T* x, int n, hipStream_t s)
+{
+    if (n <= 0) return;
+    eye_kernel<T><<<(n * n + 255) / 256, 256, 0, s>>>(x, n);
+    HIP_LAUNCH_CHECK();
+}
+
 template <typename T> void launch_fill(T* x, int64_t n, T v, hipStream_t s)
 {
     if (n <= 0) return;
@@ -1150,6 +1164,7 @@ void launch_zero_double(double* x, int n, hipStream_t s)
     template void launch_mv_phase<T>(const Phase&, const Bases&, T, T, hipStream_t);                \
     template void launch_scale<T>(T*, int64_t, T, hipStream_t);                                     \
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                      \
+    template void launch_eye<T>(T*, int, hipStream_t);                                              \
     template void launch_axpby<T>(const T*, const T*, T, T*, int64_t, hipStream_t);                 \
     template void launch_gaussian<T>(T*, int64_t, uint64_t, uint64_t, hipStream_t);                 \
     template void launch_add_diag<T>(T*, const int64_t*, const int*, int, T, hipStream_t);          \

@@ -34,6 +34,8 @@ template <typename T> void launch_fill(T* x, int64_t n, T v, hipStream_t s);
 // out = a + alpha * b (elementwise)
 template <typename T> void launch_axpby(const T* a, const T* b, T alpha, T* out, int64_t n, hipStream_t s);
 void launch_fill_int(int* x, int64_t n, int v, hipStream_t s);
+// x (n x n, ld n) = identity
+template <typename T> void launch_eye(T* x, int n, hipStream_t s);
 template <typename Ts, typename Td> void launch_convert(const Ts* x, Td* y, int64_t n, hipStream_t s);
 template <typename T> void launch_gaussian(T* x, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s);
 

@@ -1,5 +1,6 @@
 #include "plan.hpp"
 #include <algorithm>
+#include <map>
 #include <stdexcept>
 
 namespace dmhm {
@@ -473,7 +474,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     }
     auto inBatch = [&](const std::vector<char>& grp, int b) { return unit[b] >= 0 && grp[unit[b]]; };
 
-    plan->phases.reserve(3 + 17 * groups.size());
+    plan->phases.reserve(4 + 18 * groups.size());
     // Exact factor of a low-rank term: the side that is not a stored factor
     // of A or B is a sub-H product, written to a fresh workspace panel.
     auto buildW = [&](PhaseBuilder& proj, PhaseBuilder& acc, const Pending& wp) {
@@ -501,6 +502,13 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     //    T1 = B(u,s) Omega_s for the NODE*NODE terms of non-leaf blocks.
     std::unordered_map<uint64_t, int64_t> omegaOf;     // (level, s) -> panel
     std::unordered_map<uint64_t, int64_t> t1Of;        // B block -> T1 panel
+    // A 64 x 64 identity, the right operand of the gathers below.
+    const int64_t eyeOff = wsAlloc(64 * 64);
+    {
+        Phase& eye = newPhase(PH_EYE);
+        eye.zeroOff = eyeOff;
+        eye.zeroCount = 64;
+    }
     {
         Phase& rngPh = newPhase(PH_RNG);
         for (int lbk : g.lrBlocks) {
@@ -669,14 +677,75 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             s.G2 = wsAlloc((int64_t)kS * LC.cap[C.lrIndex]);
             sk.push_back(s);
         }
-        // W factors of this batch.
+        // W factors of this batch. Terms that share their sub-H block (the
+        // same B(u,s) for a low-rank A(t,u), or A(t,u) for a low-rank B(u,s))
+        // are stacked: their stored factors are gathered side by side into
+        // one panel, and one sub-H product of width cap * n computes all
+        // their exact factors at once (wider tiles, fewer contributions).
         {
+            Phase& gp = newPhase(PH_TILE);
             Phase& p1 = newPhase(PH_TILE);
             p1.role = 1;
             Phase& a1 = newPhase(PH_TILE);
-            PhaseBuilder proj(p1), acc(a1);
-            for (const auto& wp : wPending)
-                if (inBatch(inGroup, wp.bC)) buildW(proj, acc, wp);
+            PhaseBuilder gather(gp), proj(p1), acc(a1);
+            std::map<std::pair<int, int>, std::vector<int>> stacks;
+            std::vector<std::pair<int, int>> order;
+            for (size_t i = 0; i < wPending.size(); ++i) {
+                const auto& wp = wPending[i];
+                if (!inBatch(inGroup, wp.bC)) continue;
+                const std::pair<int, int> key{wp.left ? 1 : 0, wp.left ? wp.bB : wp.bA};
+                auto& v = stacks[key];
+                if (v.empty()) order.push_back(key);
+                v.push_back((int)i);
+            }
+            for (const auto& key : order) {
+                const auto& idx = stacks[key];
+                const Pending& w0 = wPending[idx[0]];
+                const int cap = W[w0.wIndex].cap;
+                // Stacking computes every column of every factor; it pays for
+                // factors of products and iterates (full rank), while imported
+                // operands keep the rank-indirect per-term form (rank-0 blocks
+                // then cost nothing).
+                bool same = cap <= 64 && (w0.left ? stackA_ : stackB_);
+                for (int i : idx) same = same && W[wPending[i].wIndex].cap == cap;
+                if (idx.size() < 2 || !same) {
+                    for (int i : idx) buildW(proj, acc, wPending[i]);
+                    continue;
+                }
+                const int n = (int)idx.size();
+                const bool left = w0.left;
+                const Block& sub = g.blocks[left ? w0.bB : w0.bA];
+                // Input rows: the u box (B's target box / A's source box).
+                const int inLevel = sub.level, inBoxId = left ? sub.t : sub.s;
+                const Box& inB = g.boxes[inLevel][inBoxId];
+                const WFactor& wf0 = W[w0.wIndex];
+                const int outBoxId = left ? wf0.s : wf0.t;
+                const Box& outB = g.boxes[wf0.level][outBoxId];
+                const int64_t IN = wsAlloc((int64_t)inB.size * cap * n);
+                const int64_t OUT = wsAlloc((int64_t)outB.size * cap * n);
+                for (int i = 0; i < n; ++i) {
+                    const Pending& wp = wPending[idx[i]];
+                    const Block& lr = g.blocks[left ? wp.bA : wp.bB];
+                    const int li = lr.lrIndex;
+                    const int64_t src = left ? LA.vOff[li] : LB.uOff[li];
+                    const uint8_t slot = left ? SLOT_A : SLOT_B;
+                    for (int a = inB.atomBegin; a < inB.atomEnd; ++a) {
+                        const int ra = g.atomStart[a] - inB.off;
+                        Tile t{};
+                        t.off = IN + ra + (int64_t)i * cap * inB.size; t.ld = inB.size;
+                        t.m = g.atomSize[a]; t.k = cap; t.mrank = t.krank = -1; t.slot = SLOT_WS;
+                        const int tile = gather.addTile(t);
+                        gather.add(tile, mkc(slot, src + ra, inB.size, SLOT_WS, eyeOff, 64, cap, -1, 0, 0, 1.0f));
+                    }
+                    WFactor& w = W[wp.wIndex];
+                    if (left) w.PV.off = OUT + (int64_t)i * cap * outB.size;
+                    else w.PU.off = OUT + (int64_t)i * cap * outB.size;
+                }
+                PanelTiles pt = addPanelTiles(acc, {SLOT_WS, OUT, outB.size}, wf0.level, outBoxId, cap * n, -1, 0, 0);
+                subh(proj, acc, left ? SLOT_B : SLOT_A, left ? LB : LA, left ? w0.bB : w0.bA, left,
+                     {SLOT_WS, IN, inB.size}, inLevel, inBoxId, pt, cap * n, -1, 0, 1.0f, 0, &plan->flops);
+            }
+            gather.finalize();
             proj.finalize();
             acc.finalize();
         }
@@ -858,7 +927,8 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     std::vector<Phase> kept;
     kept.reserve(plan->phases.size());
     for (auto& ph : plan->phases) {
-        const bool empty = ph.tiles.empty() && ph.qr.empty() && ph.svd.empty() && ph.rng.empty();
+        const bool empty = ph.kind != PH_EYE && ph.tiles.empty() && ph.qr.empty() && ph.svd.empty() &&
+                           ph.rng.empty();
         if (!empty) kept.push_back(std::move(ph));
     }
     plan->phases.swap(kept);

@@ -100,7 +100,7 @@ struct MvContrib {
     int32_t pad;
 };
 
-enum PhaseKind { PH_TILE, PH_QR, PH_SVD, PH_RNG, PH_SCALE_C, PH_MV_ZERO, PH_MV_PROJ, PH_MV_ACC };
+enum PhaseKind { PH_TILE, PH_QR, PH_SVD, PH_RNG, PH_SCALE_C, PH_MV_ZERO, PH_MV_PROJ, PH_MV_ACC, PH_EYE };
 
 struct Phase {
     PhaseKind kind;
@@ -112,7 +112,7 @@ struct Phase {
     std::vector<MvProjItem> mvProj;
     std::vector<MvAtom> mvAtoms;
     std::vector<MvContrib> mvContribs;
-    int64_t zeroOff = 0, zeroCount = 0;   // PH_MV_ZERO range in the workspace
+    int64_t zeroOff = 0, zeroCount = 0;   // PH_MV_ZERO range in the workspace; PH_EYE: offset, order
     int mvK = 0, ldx = 0, ldy = 0;
     void* dAux = nullptr;
     int qrK = 0;            // panel width of the QR phase
@@ -214,6 +214,9 @@ public:
     // Row-block height of the TSQR of tall sketch panels (0: 1024 for sketch
     // widths <= 24, else 512, the register-resident QR limits).
     void setTsqrRows(int rows) { tsqrRows_ = rows; }
+    // Whether the low-rank factors of A / B are expected at full rank, so that
+    // the exact-factor terms they enter are stacked (see hhProduct).
+    void setStacking(bool a, bool b) { stackA_ = a; stackB_ = b; }
 
     // Sub-H product: out(|outBox| x k) (+)= op(M(b)) in(|inBox| x k). Leaves of
     // M inside block b contribute to the tiles of `out` (registered per atom in
@@ -236,6 +239,7 @@ private:
     int64_t ws_ = 0;
     int64_t wsBudget_ = int64_t(1) << 31;
     int tsqrRows_ = 0;
+    bool stackA_ = true, stackB_ = true;
 };
 
 } // namespace dmhm
