@@ -99,6 +99,9 @@ SIGNATURES = {
                                            _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
     "dmhm_compress_dense": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip]),
     "dmhm_profile_hh_phases": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _ip, _ip]),
+    "dmhm_hmat_packed_size": (ctypes.c_int, [_h, ctypes.POINTER(ctypes.c_uint64)]),
+    "dmhm_hmat_pack": (ctypes.c_int, [_h, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "dmhm_hmat_unpack": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _hp]),
     "dmhm_hmat_copy": (ctypes.c_int, [_h, _hp]),
     "dmhm_hmat_transpose": (ctypes.c_int, [_h, _hp]),
     "dmhm_hmat_scale": (ctypes.c_int, [_h, ctypes.c_double]),

@@ -1417,6 +1417,147 @@ int dmhm_hmat_to_dense(dmhm_hmat h, double* out, int ld)
     });
 }
 
+// Packed form: a self-describing little-endian byte image.
+//   char[8] "DMHMAMD1"; int32 dim, sizes[3], numLevels, maxRank, strong,
+//   precision, full; int64 nLowRank, denseElems; int32 caps[nLowRank];
+//   int32 ranks[nLowRank]; dense leaves (denseElems values, tree order);
+//   then per admissible block U (m x rank) and V (n x rank), column-major.
+// Only the first `rank` columns of each factor are stored.
+static const char kPackMagic[8] = {'D', 'M', 'H', 'M', 'A', 'M', 'D', '1'};
+
+struct PackHeader {
+    char magic[8];
+    int32_t dim, sizes[3], numLevels, maxRank, strong, precision, full;
+    int64_t nLowRank, denseElems;
+};
+
+static void read_back(dmhm_hmat_s* h, std::vector<char>& vals, std::vector<int>& ranks)
+{
+    const Layout& L = *h->lay;
+    HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    vals.resize(L.totalElems() * h->esize());
+    ranks.resize(L.nLowRank());
+    if (!vals.empty()) HIPCHECK(hipMemcpy(vals.data(), h->vals, vals.size(), hipMemcpyDeviceToHost));
+    if (!ranks.empty()) HIPCHECK(hipMemcpy(ranks.data(), h->ranks, ranks.size() * sizeof(int), hipMemcpyDeviceToHost));
+}
+
+static uint64_t packed_size(dmhm_hmat_s* h, const std::vector<int>& ranks)
+{
+    const Geometry& g = h->geo();
+    const Layout& L = *h->lay;
+    uint64_t elems = L.denseElems;
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        const int b = g.lrBlocks[i];
+        elems += (uint64_t)(g.tbox(b).size + g.sbox(b).size) * std::min(ranks[i], L.cap[i]);
+    }
+    return sizeof(PackHeader) + 2 * sizeof(int32_t) * L.nLowRank() + elems * h->esize();
+}
+
+static uint64_t pack_hmat(dmhm_hmat_s* h, char* out, uint64_t cap)
+{
+    const Geometry& g = h->geo();
+    const Layout& L = *h->lay;
+    std::vector<char> vals;
+    std::vector<int> ranks;
+    read_back(h, vals, ranks);
+    const uint64_t need = packed_size(h, ranks);
+    if (!out) return need;
+    if (cap < need) throw Error(DMHM_EINVAL, "pack buffer too small");
+    PackHeader hd{};
+    std::memcpy(hd.magic, kPackMagic, 8);
+    hd.dim = g.dim;
+    for (int a = 0; a < 3; ++a) hd.sizes[a] = g.sizes[a];
+    hd.numLevels = g.numLevels; hd.maxRank = h->maxRank; hd.strong = g.strong ? 1 : 0;
+    hd.precision = h->prec; hd.full = h->full ? 1 : 0;
+    hd.nLowRank = L.nLowRank(); hd.denseElems = L.denseElems;
+    char* p = out;
+    std::memcpy(p, &hd, sizeof(hd)); p += sizeof(hd);
+    std::memcpy(p, L.cap.data(), sizeof(int32_t) * L.nLowRank()); p += sizeof(int32_t) * L.nLowRank();
+    std::memcpy(p, ranks.data(), sizeof(int32_t) * L.nLowRank()); p += sizeof(int32_t) * L.nLowRank();
+    const size_t es = h->esize();
+    std::memcpy(p, vals.data(), L.denseElems * es); p += L.denseElems * es;
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        const int b = g.lrBlocks[i];
+        const int r = std::min(ranks[i], L.cap[i]);
+        const size_t u = (size_t)g.tbox(b).size * r * es, v = (size_t)g.sbox(b).size * r * es;
+        std::memcpy(p, vals.data() + L.uOff[i] * es, u); p += u;
+        std::memcpy(p, vals.data() + L.vOff[i] * es, v); p += v;
+    }
+    return (uint64_t)(p - out);
+}
+
+static dmhm_hmat_s* unpack_hmat(const char* in, uint64_t size)
+{
+    PackHeader hd;
+    if (size < sizeof(hd)) throw Error(DMHM_EINVAL, "packed image too small");
+    std::memcpy(&hd, in, sizeof(hd));
+    if (std::memcmp(hd.magic, kPackMagic, 8) != 0) throw Error(DMHM_EINVAL, "not a packed dmhm_amd H-matrix");
+    if (hd.precision != DMHM_F64 && hd.precision != DMHM_F32) throw Error(DMHM_EINVAL, "bad precision in image");
+    auto gc = get_geo(hd.dim, hd.sizes, hd.numLevels, hd.strong != 0);
+    const Geometry& g = gc->geo;
+    if (hd.nLowRank != (int64_t)g.lrBlocks.size()) throw Error(DMHM_EINVAL, "image does not match its geometry");
+    const char* p = in + sizeof(hd);
+    const char* end = in + size;
+    auto take = [&](size_t n) {
+        if ((size_t)(end - p) < n) throw Error(DMHM_EINVAL, "packed image truncated");
+        const char* q = p; p += n; return q;
+    };
+    std::vector<int> caps(hd.nLowRank), ranks(hd.nLowRank);
+    std::memcpy(caps.data(), take(sizeof(int32_t) * hd.nLowRank), sizeof(int32_t) * hd.nLowRank);
+    std::memcpy(ranks.data(), take(sizeof(int32_t) * hd.nLowRank), sizeof(int32_t) * hd.nLowRank);
+    bool stdCaps = true;
+    for (int c : caps) stdCaps = stdCaps && c == std::max(hd.maxRank, 1);
+    std::shared_ptr<Layout> lay;
+    if (stdCaps) lay = std_layout(*gc, hd.maxRank);
+    else {
+        std::lock_guard<std::mutex> lk(g_lib.mu);
+        lay = std::make_shared<Layout>(Layout::make(g, caps, g_lib.nextLayoutId++));
+    }
+    if (lay->denseElems != hd.denseElems) throw Error(DMHM_EINVAL, "image does not match its geometry");
+    std::unique_ptr<dmhm_hmat_s> h(alloc_hmat(gc, lay, hd.precision, hd.maxRank));
+    const size_t es = h->esize();
+    std::vector<char> vals(lay->totalElems() * es, 0);
+    std::memcpy(vals.data(), take(lay->denseElems * es), lay->denseElems * es);
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        const int b = g.lrBlocks[i];
+        if (ranks[i] < 0 || ranks[i] > caps[i]) throw Error(DMHM_EINVAL, "bad rank in image");
+        const size_t u = (size_t)g.tbox(b).size * ranks[i] * es, v = (size_t)g.sbox(b).size * ranks[i] * es;
+        std::memcpy(vals.data() + lay->uOff[i] * es, take(u), u);
+        std::memcpy(vals.data() + lay->vOff[i] * es, take(v), v);
+    }
+    HIPCHECK(hipMemcpy(h->vals, vals.data(), vals.size(), hipMemcpyHostToDevice));
+    if (!ranks.empty()) HIPCHECK(hipMemcpy(h->ranks, ranks.data(), ranks.size() * sizeof(int), hipMemcpyHostToDevice));
+    h->full = hd.full != 0;
+    return h.release();
+}
+
+int dmhm_hmat_packed_size(dmhm_hmat h, uint64_t* bytes)
+{
+    return guard([&] {
+        require_ready();
+        if (!h || !bytes) throw Error(DMHM_EINVAL, "null pointer");
+        *bytes = pack_hmat(h, nullptr, 0);
+    });
+}
+
+int dmhm_hmat_pack(dmhm_hmat h, void* buf, uint64_t capacity, uint64_t* written)
+{
+    return guard([&] {
+        require_ready();
+        if (!h || !buf || !written) throw Error(DMHM_EINVAL, "null pointer");
+        *written = pack_hmat(h, static_cast<char*>(buf), capacity);
+    });
+}
+
+int dmhm_hmat_unpack(const void* buf, uint64_t size, dmhm_hmat* out)
+{
+    return guard([&] {
+        require_ready();
+        if (!buf || !out) throw Error(DMHM_EINVAL, "null pointer");
+        *out = unpack_hmat(static_cast<const char*>(buf), size);
+    });
+}
+
 int dmhm_hmat_ranks(dmhm_hmat h, int* ranks, int count)
 {
     return guard([&] {

@@ -130,6 +130,29 @@ class _HMatBase:
             ctypes.byref(out)))
         return cls(out)
 
+    @classmethod
+    def Unpack(cls, packed):
+        """Rebuild an H-matrix from Pack()'s bytes (the HMat2d(const Vector<byte>&)
+        constructor, src/HMat2d/Pack-incl.hpp)."""
+        _lib.init()
+        buf = bytes(packed)
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hmat_unpack(ctypes.c_char_p(buf), len(buf), ctypes.byref(out)))
+        return cls(out)
+
+    def PackedSize(self):
+        n = ctypes.c_uint64()
+        check(_lib.load().dmhm_hmat_packed_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def Pack(self):
+        """Packed byte image (HMat2d::Pack, src/HMat2d/Pack-incl.hpp:17-364)."""
+        n = self.PackedSize()
+        buf = ctypes.create_string_buffer(max(n, 1))
+        w = ctypes.c_uint64()
+        check(_lib.load().dmhm_hmat_pack(self._h, buf, n, ctypes.byref(w)))
+        return buf.raw[:w.value]
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value and _lib._lib is not None:

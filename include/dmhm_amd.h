@@ -222,6 +222,17 @@ typedef struct {
  * src/HMat2d/Invert-incl.hpp:128-196). Replaces h's contents. */
 int dmhm_hmat_schulz_invert(dmhm_hmat h, const dmhm_schulz_opts* opts, dmhm_schulz_report* report);
 
+/* Packed byte image of an H-matrix (HMat2d::PackedSize / Pack / Unpack and
+ * the HMat2d(const Vector<byte>&) constructor; include/dmhm/hmat2d.hpp:89,108-112,
+ * src/HMat2d/Pack-incl.hpp:27-160). Own format, not byte-compatible with the
+ * reference's: a header (magic "DMHMAMD1", geometry, maxRank, precision),
+ * the per-block capacities and ranks, the dense leaves, then each admissible
+ * block's U and V at its current rank. Unpack creates a new handle on the
+ * current GPU; pack(unpack(x)) reproduces x's image byte for byte. */
+int dmhm_hmat_packed_size(dmhm_hmat h, uint64_t* bytes);
+int dmhm_hmat_pack(dmhm_hmat h, void* buf, uint64_t capacity, uint64_t* written);
+int dmhm_hmat_unpack(const void* buf, uint64_t size, dmhm_hmat* out);
+
 /* Expand to a dense n x n column-major host matrix (HMat2d::Print multiplies
  * by the identity, HMat2d.cpp:233-248). For tests on small matrices. */
 int dmhm_hmat_to_dense(dmhm_hmat h, double* out, int ld);

@@ -27,6 +27,7 @@
 //    checks only when RELEASE is off.
 //  * The random streams differ from the reference's serial LCG: SetToRandom
 //    and the sketches draw from a counter-based generator keyed by a seed.
+//  * Pack / Unpack use this library's own byte format (not the reference's).
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -64,6 +65,8 @@ template <typename Scalar> struct precision_of;
 template <> struct precision_of<double> { static constexpr int value = DMHM_F64; };
 template <> struct precision_of<float> { static constexpr int value = DMHM_F32; };
 } // namespace detail
+
+typedef unsigned char byte;
 
 // Environment (include/dmhm/core/environment.hpp:38-39): binds GPU 0.
 inline void Initialize(int& /*argc*/, char**& /*argv*/) { detail::check(dmhm_init(0)); }
@@ -284,6 +287,35 @@ public:
     }
     const dmhm_schulz_report& LastSchulzReport() const { return report_; }
 
+    // Packed byte image (include/dmhm/hmat2d.hpp:89,108-112; own format, see
+    // dmhm_hmat_pack).
+    std::size_t PackedSize() const
+    {
+        uint64_t n = 0;
+        detail::check(dmhm_hmat_packed_size(need(), &n));
+        return (std::size_t)n;
+    }
+    std::size_t Pack(byte* packedHMat) const
+    {
+        uint64_t w = 0;
+        detail::check(dmhm_hmat_pack(need(), packedHMat, PackedSize(), &w));
+        return (std::size_t)w;
+    }
+    std::size_t Pack(Vector<byte>& packedHMat) const
+    {
+        packedHMat.Resize((int)PackedSize());
+        return Pack(packedHMat.Buffer());
+    }
+    std::size_t Unpack(const byte* packedHMat, std::size_t size)
+    {
+        replace([&](dmhm_hmat* out) { return dmhm_hmat_unpack(packedHMat, size, out); });
+        return size;
+    }
+    std::size_t Unpack(const Vector<byte>& packedHMat)
+    {
+        return Unpack(packedHMat.LockedBuffer(), (std::size_t)packedHMat.Size());
+    }
+
 protected:
     dmhm_hmat h_ = nullptr;
     dmhm_schulz_report report_{};
@@ -367,6 +399,7 @@ public:
         detail::check(dmhm_build_natural_to_hierarchical_map(2, sizes, numLevels, map.Buffer()));
     }
     HMat2d() = default;
+    explicit HMat2d(const Vector<byte>& packedHMat) { this->Unpack(packedHMat); }
     HMat2d(int numLevels, int maxRank, bool symmetric, bool stronglyAdmissible, int xSize, int ySize)
     {
         const int sizes[2] = {xSize, ySize};
@@ -397,6 +430,7 @@ public:
         detail::check(dmhm_build_natural_to_hierarchical_map(3, sizes, numLevels, map.Buffer()));
     }
     HMat3d() = default;
+    explicit HMat3d(const Vector<byte>& packedHMat) { this->Unpack(packedHMat); }
     HMat3d(int numLevels, int maxRank, bool symmetric, bool stronglyAdmissible, int xSize, int ySize, int zSize)
     {
         const int sizes[3] = {xSize, ySize, zSize};

@@ -224,6 +224,16 @@ static int run_gpu()
     e = rel_diff(yf, ref, m);
     expect(e < 1e-5, "HMat2d<float>::Multiply", e);
 
+    // Pack / Unpack round trip (tests/HMat2d/Pack.cpp of the reference).
+    Vector<byte> image;
+    X64.Pack(image);
+    HMat2d<double> Xr(image);
+    Vector<byte> image2;
+    Xr.Pack(image2);
+    bool same = image.Size() == image2.Size();
+    for (int i = 0; same && i < image.Size(); ++i) same = image[i] == image2[i];
+    expect(same && (std::size_t)image.Size() == X64.PackedSize(), "Pack / Unpack round trip", image.Size());
+
     // Error mapping: nonconformal H*H raises std::logic_error.
     bool threw = false;
     try {

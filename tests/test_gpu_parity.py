@@ -418,3 +418,38 @@ def test_compress_matches_oracle(m, n, max_rank):
     Ur, Vr = orc.compress_dense(max_rank, D)
     assert U.shape[1] == Ur.shape[1] == min(max_rank, m, n)
     assert _relmax(U @ V.T, Ur @ Vr.T) < 1e-11
+
+
+# --- Pack / Unpack ----------------------------------------------------------
+
+@pytest.mark.parametrize("kind", ["import", "random64", "random32", "product"])
+def test_pack_unpack_round_trip(kind):
+    """Unpack(Pack(H)) reproduces H exactly and packs to the same bytes (the
+    reference's own hard check, tests/HMat2d/Pack.cpp:217)."""
+    op = W.small_operator(16, 3)
+    H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 8, False, *_args(op))
+    if kind.startswith("random"):
+        H.SetToRandom(seed=3)
+        if kind == "random32":
+            H = H.Convert(dmhm_amd.DMHM_F32)
+    elif kind == "product":
+        H.SetToRandom(seed=4)
+        H = H.Multiply(1.0, H)
+    img = H.Pack()
+    assert len(img) == H.PackedSize()
+    G = dmhm_amd.HMat2d.Unpack(img)
+    assert np.array_equal(G.to_dense(), H.to_dense())
+    assert np.array_equal(G.ranks(), H.ranks())
+    assert G.Pack() == img
+    x = np.random.default_rng(0).standard_normal(256)
+    assert np.array_equal(G.Multiply(1.0, x), H.Multiply(1.0, x))
+
+
+def test_unpack_rejects_bad_images():
+    op = W.small_operator(16, 3)
+    H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 8, True, *_args(op))
+    img = H.Pack()
+    with pytest.raises(dmhm_amd.DmhmError):
+        dmhm_amd.HMat2d.Unpack(img[:-8])
+    with pytest.raises(dmhm_amd.DmhmError):
+        dmhm_amd.HMat2d.Unpack(b"XXXXXXXX" + img[8:])
