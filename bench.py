@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--mode", choices=["full", "matvec", "hh"], default="full",
                     help="matvec / hh: only the kernel loop (for rocprofv3 runs)")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--config", choices=["c2", "c4"], default="c2",
+                    help="c2: 2D 256^2 (the headline workload); c4: 2D 512^2, sigma=1.5, l=0.05, "
+                         "numLevels 7, maxRank 32 (kernel modes only)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU; there is no multi-GPU path in this build (requested --gpus %d)" % a.gpus,
@@ -119,8 +122,11 @@ def main():
     from dmhm_amd import workloads as W
 
     dmhm_amd.init(0)
-    log("bench: building the 256^2 lognormal operator")
-    cfg = W.config_operator("c2", seed=0)
+    if a.config == "c4" and a.mode == "full":
+        print("the 512^2 workload is measured in the kernel modes (--mode matvec / hh)", file=sys.stderr)
+        sys.exit(2)
+    log("bench: building the %s operator" % ("256^2 lognormal" if a.config == "c2" else "512^2 lognormal"))
+    cfg = W.config_operator(a.config, seed=0)
     args = (cfg["rowptr"], cfg["colind"], cfg["vals"])
     L, r, n = cfg["num_levels"], cfg["max_rank"], int(np.prod(cfg["sizes"]))
     A = dmhm_amd.HMat2d.from_csr(cfg["sizes"], L, r, True, *args)
@@ -133,7 +139,8 @@ def main():
         # known-size launch that scales the FETCH_SIZE / WRITE_SIZE counters.
         X32 = X.Convert(dmhm_amd.DMHM_F32)
         ms, by = X32.profile_matvec(1, a.reps)
-        print(json.dumps({"mode": "matvec", "ms": ms.tolist(), "bytes": by.tolist(),
+        print(json.dumps({"mode": "matvec", "config": a.config, "ms": ms.tolist(), "bytes": by.tolist(),
+                          "GBps": float(by.sum() / (ms[0] * 1e-3) / 1e9),
                           "convert_elements": int(X.info().storageBytes // 8)}))
         return
     if a.mode == "hh":
@@ -143,8 +150,14 @@ def main():
         Y.SetToRandom(seed=2)
         for _ in range(a.reps):
             ms, fl = X.profile_hh(Y)
-        print(json.dumps({"mode": "hh", "ms": ms.tolist(), "flops": fl.tolist(),
-                          "phases": X.last_hh_phases()}))
+        out = {"mode": "hh", "config": a.config, "ms": ms.tolist(), "flops": fl.tolist(),
+               "phases": X.last_hh_phases(cap=4096)}
+        if a.config == "c4":
+            X32, Y32 = X.Convert(dmhm_amd.DMHM_F32), Y.Convert(dmhm_amd.DMHM_F32)
+            ms32, fl32 = X32.profile_hh(Y32)
+            out["f32_ms"] = ms32.tolist()
+            out["f32_flops"] = fl32.tolist()
+        print(json.dumps(out))
         return
 
     # Warm-up: plan building and workspace. Two iterations in each precision
