@@ -604,26 +604,26 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
     for (int j = 0; j < KMAX; ++j) {
         if (j < kk) {
             const int buf = j & 1;
-            double p[KMAX];
-#pragma unroll
-            for (int c = j; c < KMAX; ++c) p[c] = 0.0;
 #pragma unroll
             for (int r = 0; r < RPL; ++r) {
                 const int i = tid + NT * r;
-                if (i > j && i < m) {
-                    const double x = a[r][j];
-#pragma unroll
-                    for (int c = j; c < KMAX; ++c) p[c] += x * a[r][c];
-                }
                 if (i == j) {
 #pragma unroll
                     for (int c = j; c < KMAX; ++c) rowj[buf][c] = a[r][c];
                 }
             }
+            // Column products reduced one column at a time (no partial array:
+            // keeps the wave within 128 registers at KMAX = 40).
 #pragma unroll
             for (int c = j; c < KMAX; ++c) {
                 if (c < k) {
-                    const double sum = wave_sum(p[c]);
+                    double pc = 0.0;
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) {
+                        const int i = tid + NT * r;
+                        if (i > j && i < m) pc += a[r][j] * a[r][c];
+                    }
+                    const double sum = wave_sum(pc);
                     if (lane == 0) part[buf][w][c] = sum;
                 }
             }
@@ -687,20 +687,17 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
         if (j < kk) {
             const int buf = j & 1;
             const double t = taus[j];
-            double p[KMAX];
-#pragma unroll
-            for (int c = j + 1; c < KMAX; ++c) p[c] = 0.0;
-#pragma unroll
-            for (int r = 0; r < RPL; ++r) {
-                const int i = tid + NT * r;
-                const double v = (i == j) ? 1.0 : ((i > j && i < m) ? a[r][j] : 0.0);
-#pragma unroll
-                for (int c = j + 1; c < KMAX; ++c) p[c] += v * a[r][c];
-            }
 #pragma unroll
             for (int c = j + 1; c < KMAX; ++c) {
                 if (c < k) {
-                    const double sum = wave_sum(p[c]);
+                    double pc = 0.0;
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) {
+                        const int i = tid + NT * r;
+                        const double v = (i == j) ? 1.0 : ((i > j && i < m) ? a[r][j] : 0.0);
+                        pc += v * a[r][c];
+                    }
+                    const double sum = wave_sum(pc);
                     if (lane == 0) part[buf][w][c] = sum;
                 }
             }
