@@ -1,12 +1,13 @@
 """Synthetic operators for the H-matrix path, generated on the host in FP64.
 
 The same bytes feed the GPU library and the CPU oracle. The stencil is the
-flux form of the reference driver (tests/HMat2d/Invert.cpp:12-67): the
+flux form of the reference's inversion test program
+(tests/HMat2d/Invert.cpp:12-67): the
 coefficient lives on the (n+2)^d node grid including the Dirichlet boundary
 nodes, each face coefficient is the mean of the two node values it joins,
 off-diagonals are -face/h^2 and the diagonal is the sum of the face
-coefficients / h^2. Two deliberate differences from that driver: h = 1/(n+1)
-(the driver uses 1/n) and a +u mass term (+1 on the diagonal; the driver's
+coefficients / h^2. Two deliberate differences from that program:
+h = 1/(n+1) (it uses 1/n) and a +u mass term (+1 on the diagonal; its
 DomainV is zero). Rows and columns are in the hierarchical ordering of
 HMat2d::BuildNaturalToHierarchicalMap (src/HMat2d/HMat2d.cpp:27-87) /
 HMat3d (src/HMat3d/HMat3d.cpp:28-90).
