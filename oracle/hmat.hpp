@@ -19,6 +19,10 @@
 //  H*H product ............... dense product of the expanded operands
 //  EstimateTwoNorm, Schulz ... dense 2-norms, quadratic convergence
 //  LCG / Box-Muller .......... Python big-integer closed form
+// Parity unpinned (no reference build: MPI is absent; the reference holds no
+// golden vectors for these): bitwise identity with the reference of the
+// sketch outputs (Gaussian draw order, pivot order, SVD signs) and of the
+// Schulz iterate sequence. Only the properties above are checked for them.
 #pragma once
 #include "la.hpp"
 #include "rng.hpp"
