@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -183,6 +184,20 @@ static void run_plan(Plan& p, const Bases& b, T alpha, T beta, TruncPolicy pol, 
     }
 }
 
+// Workspace budget (elements) of one H*H batch: 2^33 (64 GiB in FP64) by
+// default, which keeps the 256^2 rank-16 product in 2-3 batches (103 ms FP64,
+// against 116 ms with 2^31 and 185 ms with 2^29); DMHM_HH_BATCH_ELEMS
+// overrides it (plans built before a change keep theirs).
+static int64_t hh_batch_budget()
+{
+    static const int64_t v = [] {
+        const char* e = getenv("DMHM_HH_BATCH_ELEMS");
+        const long long x = e ? atoll(e) : 0;
+        return x > 0 ? (int64_t)x : (int64_t(1) << 33);
+    }();
+    return v;
+}
+
 static Plan& get_plan(GeoCtx& gc, const std::string& key, const std::function<Plan*()>& make)
 {
     auto it = gc.plans.find(key);
@@ -234,6 +249,7 @@ static dmhm_hmat_s* multiply_hmat(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, 
                             std::to_string(over) + ":" + std::to_string(A->full) + std::to_string(B->full);
     Plan& p = get_plan(gc, key, [&] {
         Planner pl(gc.geo);
+        pl.setWorkspaceBudget(hh_batch_budget());
         pl.setStacking(A->full, B->full);
         return pl.hhProduct(*A->lay, *B->lay, *LC, maxRank, over);
     });
@@ -264,6 +280,7 @@ static void multiply_hmat_update(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, d
                             ":" + std::to_string(A->full) + std::to_string(B->full);
     Plan& p = get_plan(gc, key, [&] {
         Planner pl(gc.geo);
+        pl.setWorkspaceBudget(hh_batch_budget());
         pl.setStacking(A->full, B->full);
         return pl.hhProduct(*A->lay, *B->lay, *C->lay, maxRank, over, Planner::HH_UPDATE);
     });
@@ -987,6 +1004,7 @@ static void profile_hh(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, const dmhm_
                             ":" + std::to_string(A->full) + std::to_string(B->full);
     Plan& p = get_plan(gc, key, [&] {
         Planner pl(gc.geo);
+        pl.setWorkspaceBudget(hh_batch_budget());
         pl.setStacking(A->full, B->full);
         return pl.hhProduct(*A->lay, *B->lay, *LC, maxRank, over);
     });
