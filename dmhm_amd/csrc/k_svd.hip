@@ -1,0 +1,131 @@
+// gfx950 batched one-sided Jacobi SVD of small square factors.
+#include "kcommon.hpp"
+
+namespace dmhm {
+
+// ---------------------------------------------------------------------------
+// Batched one-sided Jacobi SVD of k x k (k <= 64), FP64 in LDS.
+// R = U S V^T; outputs G1 = V S (first `keep` columns, rest zero) and
+// G2 = U (same), both k x cap, and the kept rank.
+// ---------------------------------------------------------------------------
+// KMAX bounds k (LDS sized for it); NT threads, 8 per column pair.
+template <typename T, int KMAX, int NT>
+__global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restrict__ tasks,
+                                                 int tolMode, double relTol)
+{
+    static_assert(NT >= 8 * (KMAX / 2), "one 8-thread group per column pair");
+    __shared__ double As[KMAX][KMAX + 1];
+    __shared__ double Vs[KMAX][KMAX + 1];
+    __shared__ double sig[KMAX];
+    __shared__ int order[KMAX];
+    __shared__ int rotated;
+    __shared__ int keep_s;
+    const SvdTask tk = tasks[blockIdx.x];
+    const int k = tk.k, tid = threadIdx.x;
+    T* ws = reinterpret_cast<T*>(b.val[SLOT_WS]);
+    for (int e = tid; e < k * k; e += NT) {
+        const int r = e % k, c = e / k;
+        As[c][r] = (double)ws[tk.roff + e];
+        Vs[c][r] = (r == c) ? 1.0 : 0.0;
+    }
+    const int kp = k + (k & 1);
+    const int pi = tid >> 3, sub = tid & 7;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        __syncthreads();
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int round = 0; round < kp - 1; ++round) {
+            if (pi < kp / 2) {
+                auto pos = [&](int x) { return x == 0 ? 0 : 1 + ((x - 1 + round) % (kp - 1)); };
+                const int a0 = pos(pi), a1 = pos(kp - 1 - pi);
+                const int p = min(a0, a1), q = max(a0, a1);
+                double al = 0, be = 0, ga = 0;
+                if (q < k)
+                    for (int r = sub; r < k; r += 8) {
+                        const double x = As[p][r], y = As[q][r];
+                        al += x * x; be += y * y; ga += x * y;
+                    }
+#pragma unroll
+                for (int off = 4; off >= 1; off >>= 1) {
+                    al += __shfl_xor(al, off, 64);
+                    be += __shfl_xor(be, off, 64);
+                    ga += __shfl_xor(ga, off, 64);
+                }
+                if (q < k && ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
+                    const double zeta = (be - al) / (2.0 * ga);
+                    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                    for (int r = sub; r < k; r += 8) {
+                        const double x = As[p][r], y = As[q][r];
+                        As[p][r] = cs * x - sn * y; As[q][r] = sn * x + cs * y;
+                        const double vx = Vs[p][r], vy = Vs[q][r];
+                        Vs[p][r] = cs * vx - sn * vy; Vs[q][r] = sn * vx + cs * vy;
+                    }
+                    if (sub == 0) rotated = 1;
+                }
+            }
+            __syncthreads();
+        }
+        const int rot = rotated;
+        if (!rot) break;
+    }
+    __syncthreads();
+    if (tid < k) {
+        double s = 0;
+        for (int r = 0; r < k; ++r) s += As[tid][r] * As[tid][r];
+        sig[tid] = sqrt(s);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int c = 0; c < k; ++c) order[c] = c;
+        for (int a = 0; a < k; ++a) {          // selection sort, descending
+            int best = a;
+            for (int c = a + 1; c < k; ++c)
+                if (sig[order[c]] > sig[order[best]]) best = c;
+            const int tmp = order[a]; order[a] = order[best]; order[best] = tmp;
+        }
+        int keep = tk.maxKeep;
+        if (tolMode == 1) {
+            const double s0 = sig[order[0]];
+            if (s0 < relTol * relTol) keep = min(1, keep);
+            else {
+                int r = 0;
+                while (r < keep && sig[order[r]] > relTol * s0) ++r;
+                keep = r;
+            }
+        }
+        keep_s = keep;
+        b.rank[SLOT_C][tk.rankIndex] = keep;
+    }
+    __syncthreads();
+    const int keep = keep_s;
+    for (int e = tid; e < k * tk.cap; e += NT) {
+        const int r = e % k, j = e / k;
+        double g1 = 0.0, g2 = 0.0;
+        if (j < keep) {
+            const int c = order[j];
+            g1 = Vs[c][r] * sig[c];
+            g2 = sig[c] > 0 ? As[c][r] / sig[c] : 0.0;
+        }
+        ws[tk.g1off + e] = (T)g1;
+        ws[tk.g2off + e] = (T)g2;
+    }
+}
+
+template <typename T>
+void launch_svd_phase(const Phase& ph, const Bases& b, TruncPolicy pol, hipStream_t s)
+{
+    const int n = (int)ph.svd.size();
+    if (n == 0) return;
+    const SvdTask* t = static_cast<const SvdTask*>(ph.dTasks);
+    if (ph.svdK > 64) throw std::runtime_error("SVD factor wider than 64");
+    if (ph.svdK <= 24) svd_kernel<T, 24, 128><<<n, 128, 0, s>>>(b, t, pol.tolMode, pol.relTol);
+    else if (ph.svdK <= 40) svd_kernel<T, 40, 192><<<n, 192, 0, s>>>(b, t, pol.tolMode, pol.relTol);
+    else svd_kernel<T, 64, 256><<<n, 256, 0, s>>>(b, t, pol.tolMode, pol.relTol);
+    HIP_LAUNCH_CHECK();
+}
+
+template void launch_svd_phase<double>(const Phase&, const Bases&, TruncPolicy, hipStream_t);
+template void launch_svd_phase<float>(const Phase&, const Bases&, TruncPolicy, hipStream_t);
+
+} // namespace dmhm
