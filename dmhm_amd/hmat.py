@@ -223,6 +223,10 @@ class _HMatBase:
                 raise ValueError("torch operands must be column-major (use .t().contiguous().t())")
             ldx = Xc.stride(1) if k > 1 else n
             ldy = Yc.stride(1) if k > 1 else n
+            # The library works on its own stream: finish torch's pending work on
+            # X and Y (e.g. the fill of a fresh Y) before the library reads or
+            # writes them. The call returns after the library's work is done.
+            torch.cuda.current_stream(X.device).synchronize()
             check(_lib.load().dmhm_hmat_multiply_dense(self._h, op.encode(), alpha, ctypes.c_void_p(Xc.data_ptr()),
                                                        ldx, k, beta, ctypes.c_void_p(Yc.data_ptr()), ldy,
                                                        DMHM_DEVICE))
@@ -235,10 +239,19 @@ class _HMatBase:
             Yc = np.zeros((n, k), dtype=dt, order="F")
             beta = 0.0
         else:
+            if not isinstance(Y, np.ndarray) or Y.size != n * k:
+                raise ValueError("Y must be a numpy array with n x k entries")
             Yc = np.asfortranarray(np.reshape(Y, (n, -1)), dtype=dt)
         check(_lib.load().dmhm_hmat_multiply_dense(self._h, op.encode(), alpha,
                                                    Xc.ctypes.data_as(ctypes.c_void_p), n, k, beta,
                                                    Yc.ctypes.data_as(ctypes.c_void_p), n, DMHM_HOST))
+        if Y is not None:
+            # The reference updates y in place (MultiplyVector-incl.hpp:12-64):
+            # write the result back when Y was not already a Fortran-ordered
+            # array of the matrix's precision.
+            if not np.shares_memory(Yc, Y):
+                Y[...] = Yc.reshape(Y.shape)
+            return Y
         return Yc.reshape(-1) if vec else Yc
 
     def Multiply(self, alpha, B, beta=None, C=None, **hh_options):

@@ -15,7 +15,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_LAPACK_PATH = os.path.join(_HERE, "liboracle_lapack.so")
 _lib = None
+_lapack = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -30,11 +32,26 @@ def build():
 
 
 def lib():
+    """The self-contained oracle (textbook dense routines): the checker."""
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
+    return _lib
+
+
+def lapack_lib():
+    """The same oracle with its dense routines bound to MKL's sequential
+    BLAS/LAPACK (liboracle_lapack.so), or None when it was not built (no MKL)."""
+    global _lapack
+    if _lapack is None and os.path.exists(_LAPACK_PATH):
+        _lapack = _bind(ctypes.CDLL(_LAPACK_PATH, mode=ctypes.RTLD_GLOBAL))  # MKL loads its CPU kernels by name
+    return _lapack
+
+
+def _bind(L):
+    if L is not None:
         sig = {
             "orc_set_options": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
             "orc_rng_initialize": (None, []),
@@ -78,13 +95,13 @@ def lib():
             "orc_compress_dense": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip]),
             "orc_randomized_multiply": (None, [ctypes.c_int, ctypes.c_double, _vp, _vp, ctypes.c_int,
                                                _dp, _dp, _ip]),
+            "orc_hmat_multiply_child_row": (_vp, [ctypes.c_double, _vp, _vp, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+    return L
 
 
 def _d(a):
@@ -200,32 +217,35 @@ def compress_dense(max_rank, D):
 
 
 class HMat:
-    """Owning handle on an oracle H-matrix (dim 2 or 3)."""
+    """Owning handle on an oracle H-matrix (dim 2 or 3). ``L`` is the oracle
+    library that owns it (``lib()`` by default, ``lapack_lib()`` for timing)."""
 
-    def __init__(self, ptr):
+    def __init__(self, ptr, L=None):
         self.ptr = ptr
+        self.L = L if L is not None else lib()
 
     @classmethod
-    def from_csr(cls, dim, sizes, num_levels, max_rank, strong, rowptr, colind, vals):
+    def from_csr(cls, dim, sizes, num_levels, max_rank, strong, rowptr, colind, vals, L=None):
+        L = L if L is not None else lib()
         sizes = np.array(list(sizes) + [1] * (3 - len(sizes)), dtype=np.int32)
         rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
         colind = np.ascontiguousarray(colind, dtype=np.int32)
         vals = np.ascontiguousarray(vals, dtype=np.float64)
-        p = lib().orc_hmat_create(dim, _i(sizes), num_levels, max_rank, 1 if strong else 0,
-                                  _i(rowptr), _i(colind), _d(vals))
-        return cls(p)
+        p = L.orc_hmat_create(dim, _i(sizes), num_levels, max_rank, 1 if strong else 0,
+                              _i(rowptr), _i(colind), _d(vals))
+        return cls(p, L)
 
     def __del__(self):
-        if getattr(self, "ptr", None) and _lib is not None:
-            _lib.orc_hmat_free(self.ptr)
+        if getattr(self, "ptr", None) and getattr(self, "L", None) is not None:
+            self.L.orc_hmat_free(self.ptr)
             self.ptr = None
 
     @property
     def n(self):
-        return lib().orc_hmat_height(self.ptr)
+        return self.L.orc_hmat_height(self.ptr)
 
     def set_to_random(self):
-        lib().orc_hmat_set_to_random(self.ptr)
+        self.L.orc_hmat_set_to_random(self.ptr)
 
     def multiply(self, X, alpha=1.0, trans=False):
         X = np.asfortranarray(X, dtype=np.float64)
@@ -233,53 +253,57 @@ class HMat:
             X = X.reshape(-1, 1, order="F")
         k = X.shape[1]
         Y = np.zeros((self.n, k), order="F")
-        lib().orc_hmat_multiply_dense(self.ptr, 1 if trans else 0, alpha, _d(X), X.shape[0], k,
+        self.L.orc_hmat_multiply_dense(self.ptr, 1 if trans else 0, alpha, _d(X), X.shape[0], k,
                                       0.0, _d(Y), self.n)
         return Y
 
     def to_dense(self):
         n = self.n
         out = np.zeros((n, n), order="F")
-        lib().orc_hmat_to_dense(self.ptr, _d(out))
+        self.L.orc_hmat_to_dense(self.ptr, _d(out))
         return out
 
     def count_blocks(self, max_levels=12):
         d = np.zeros(max_levels, dtype=np.int64)
         f = np.zeros(max_levels, dtype=np.int64)
-        lib().orc_hmat_count_blocks(self.ptr, d.ctypes.data_as(_lp), f.ctypes.data_as(_lp), max_levels)
+        self.L.orc_hmat_count_blocks(self.ptr, d.ctypes.data_as(_lp), f.ctypes.data_as(_lp), max_levels)
         return d, f
 
     def rank_stats(self):
         out = np.zeros(3, dtype=np.int64)
-        lib().orc_hmat_rank_stats(self.ptr, out.ctypes.data_as(_lp))
+        self.L.orc_hmat_rank_stats(self.ptr, out.ctypes.data_as(_lp))
         return {"sum_rank": int(out[0]), "max_rank": int(out[1]), "storage": int(out[2])}
 
     def matmul(self, other, alpha=1.0):
-        return HMat(lib().orc_hmat_multiply(alpha, self.ptr, other.ptr))
+        return HMat(self.L.orc_hmat_multiply(alpha, self.ptr, other.ptr), self.L)
+
+    def matmul_child_row(self, other, t, alpha=1.0):
+        """Children (t, *) of alpha self other only (a timing sample)."""
+        return HMat(self.L.orc_hmat_multiply_child_row(alpha, self.ptr, other.ptr, t), self.L)
 
     def multiply_update(self, alpha, A, B, beta):
         """self := alpha A B + beta self (in place)."""
-        lib().orc_hmat_multiply_update(alpha, A.ptr, B.ptr, beta, self.ptr)
+        self.L.orc_hmat_multiply_update(alpha, A.ptr, B.ptr, beta, self.ptr)
         return self
 
     def copy(self):
-        return HMat(lib().orc_hmat_copy(self.ptr))
+        return HMat(self.L.orc_hmat_copy(self.ptr), self.L)
 
     def transpose(self):
-        return HMat(lib().orc_hmat_transpose(self.ptr))
+        return HMat(self.L.orc_hmat_transpose(self.ptr), self.L)
 
     def scale(self, a):
-        lib().orc_hmat_scale(self.ptr, a)
+        self.L.orc_hmat_scale(self.ptr, a)
 
     def add_diag(self, a):
-        lib().orc_hmat_add_diag(self.ptr, a)
+        self.L.orc_hmat_add_diag(self.ptr, a)
 
     def estimate_two_norm(self, theta=1.5, confidence=6.0):
-        return lib().orc_estimate_two_norm(self.ptr, theta, confidence)
+        return self.L.orc_estimate_two_norm(self.ptr, theta, confidence)
 
     def schulz_invert(self, num_iterations=-1, theta=1.5, confidence=6.0, stop_tol=1e-4,
                       max_iter=30):
         hist = np.zeros(max(max_iter, 64))
-        it = lib().orc_hmat_schulz(self.ptr, num_iterations, theta, confidence, stop_tol,
+        it = self.L.orc_hmat_schulz(self.ptr, num_iterations, theta, confidence, stop_tol,
                                    max_iter, _d(hist), len(hist))
         return it, [h for h in hist if h >= 0]

@@ -78,6 +78,13 @@ void* orc_hmat_create(int dim, const int* sizes, int numLevels, int maxRank, int
     return h.release();
 }
 void orc_hmat_free(void* h) { delete static_cast<HMat*>(h); }
+// Children C(t, *) of C := alpha A B (timing sample of a product).
+void* orc_hmat_multiply_child_row(double alpha, void* A, void* B, int t)
+{
+    auto C = std::make_unique<HMat>();
+    HMat::multiply_child_row(alpha, *static_cast<HMat*>(A), *static_cast<HMat*>(B), *C, t);
+    return C.release();
+}
 int orc_hmat_height(void* h) { return static_cast<HMat*>(h)->height(); }
 void orc_hmat_set_to_random(void* h) { static_cast<HMat*>(h)->set_to_random(); }
 void orc_hmat_multiply_dense(void* h, int trans, double alpha, const double* X, int ldx, int k,

@@ -94,6 +94,10 @@ public:
     static void multiply(double alpha, const HMat& A, const HMat& B, HMat& C);
     static void multiply_update(double alpha, const HMat& A, const HMat& B,
                                 double beta, HMat& C);
+    // Only the children C(t, 0..nc-1) of C := alpha A B, for two NODE operands
+    // with a NODE product (the rest of C stays empty): a bounded sample of the
+    // product's work, used to time the CPU baseline.
+    static void multiply_child_row(double alpha, const HMat& A, const HMat& B, HMat& C, int t);
 
     // Schulz inversion of *this (Invert-incl.hpp:128-196). history receives the
     // per-iteration residual estimates; returns the number of iterations run.

@@ -101,6 +101,42 @@ def test_matvec_device_pointers():
     assert _relmax(Yd.cpu().numpy(), H.Multiply(1.0, X)) < 1e-14
 
 
+def test_matvec_device_operands_from_torch_kernels():
+    """X produced by a torch kernel on the device and Y freshly allocated by
+    the wrapper: the library stream must see both finished."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("torch sees no GPU")
+    op = W.small_operator(16, 3)
+    H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 4, True, *_args(op))
+    g = torch.Generator(device="cuda").manual_seed(5)
+    for k in (1, 3):
+        Xd = torch.randn((k, 256), dtype=torch.float64, device="cuda", generator=g).t()
+        Xd = Xd * 3.0 + 1.0                        # a second pending kernel on torch's stream
+        Yd = H.Multiply(1.0, Xd)
+        assert _relmax(Yd.cpu().numpy(), H.Multiply(1.0, Xd.cpu().numpy())) < 1e-14
+
+
+def test_multiply_writes_numpy_y_in_place():
+    """Multiply(alpha, X, beta, Y) updates Y in place also when Y is C-ordered
+    or of another dtype (the wrapper copies and writes back)."""
+    op = W.small_operator(16, 3)
+    H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 4, True, *_args(op))
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((256, 3))
+    Y0 = rng.standard_normal((256, 3))
+    want = 2.0 * H.Multiply(1.0, X) + 0.5 * Y0
+    for Y in (np.array(Y0, order="C"), np.array(Y0, order="F"), np.array(Y0, dtype=np.float32)):
+        out = H.Multiply(2.0, X, 0.5, Y)
+        assert out is Y
+        tol = 1e-6 if Y.dtype == np.float32 else 1e-13
+        assert _relmax(Y.astype(np.float64), want) < tol
+    y = rng.standard_normal(256)
+    want1 = H.Multiply(1.0, X[:, 0]) - y
+    H.Multiply(1.0, X[:, 0], -1.0, y)
+    assert _relmax(y, want1) < 1e-13
+
+
 def test_set_to_random_ranks_and_consistency():
     op = W.small_operator(32, 3)
     H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 6, True, *_args(op))
