@@ -5,8 +5,9 @@ the C-ABI in include/dmhm_amd.h); this package is the Python host mirror of
 the reference interface plus the synthetic workload generators.
 """
 from ._lib import (DMHM_F64, DMHM_F32, DmhmError, NonConformalError, NotConvergedError,  # noqa: F401
-                   init, load)
-from .hmat import (HMat2d, HMat3d, EstimateTwoNorm, RoundedUpdate, Compress,  # noqa: F401
+                   init, load, set_tile_engine)
+from .hmat import (HMat2d, HMat3d, EstimateTwoNorm, RoundedUpdate, Compress, RankModel,  # noqa: F401
+                   last_product_stats,
                    build_natural_to_hierarchical_map, geometry_block_counts, geometry_product_terms,
                    geometry_leaves)
 from . import workloads  # noqa: F401

@@ -15,6 +15,7 @@ DMHM_F64 = 8
 DMHM_F32 = 4
 DMHM_HOST = 0
 DMHM_DEVICE = 1
+DMHM_RANK_FEATURES = 12
 
 _ERRORS = {1: "DMHM_EINVAL", 2: "DMHM_ENONCONF", 3: "DMHM_ESINGULAR", 4: "DMHM_ENOCONV",
            5: "DMHM_EHIP", 6: "DMHM_ENOMEM", 7: "DMHM_ENODEV"}
@@ -44,25 +45,28 @@ class Info(ctypes.Structure):
 
 class HHOpts(ctypes.Structure):
     _fields_ = [("oversample", ctypes.c_int), ("tolMode", ctypes.c_int), ("relTol", ctypes.c_double),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("rankModel", ctypes.c_void_p), ("rankMargin", ctypes.c_double)]
 
 
 class SchulzOpts(ctypes.Structure):
     _fields_ = [("numIterations", ctypes.c_int), ("maxIterations", ctypes.c_int),
                 ("theta", ctypes.c_double), ("confidence", ctypes.c_double),
                 ("stopTol", ctypes.c_double), ("mixed", ctypes.c_int), ("switchTol", ctypes.c_double),
-                ("hh", HHOpts), ("stallStop", ctypes.c_int)]
+                ("hh", HHOpts), ("stallStop", ctypes.c_int), ("graph", ctypes.c_int), ("bf16", ctypes.c_int),
+                ("bf16Tol", ctypes.c_double)]
 
 
 class SchulzReport(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("iterationsLowPrecision", ctypes.c_int),
                 ("residualEstimate", ctypes.c_double), ("normEstimate", ctypes.c_double),
-                ("seconds", ctypes.c_double), ("history", ctypes.c_double * 128)]
+                ("seconds", ctypes.c_double), ("history", ctypes.c_double * 128),
+                ("graphIterations", ctypes.c_int), ("iterationsBf16", ctypes.c_int)]
 
 
 _ip = ctypes.POINTER(ctypes.c_int)
 _dp = ctypes.POINTER(ctypes.c_double)
 _llp = ctypes.POINTER(ctypes.c_longlong)
+_fp = ctypes.POINTER(ctypes.c_float)
 _h = ctypes.c_void_p
 _hp = ctypes.POINTER(ctypes.c_void_p)
 
@@ -99,6 +103,19 @@ SIGNATURES = {
                                            _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
     "dmhm_compress_dense": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip]),
     "dmhm_profile_hh_phases": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _ip, _ip]),
+    "dmhm_set_tile_engine": (ctypes.c_int, [ctypes.c_int]),
+    "dmhm_rank_model_create": (ctypes.c_int, [ctypes.c_uint64, _hp]),
+    "dmhm_rank_model_destroy": (ctypes.c_int, [_h]),
+    "dmhm_rank_model_num_params": (ctypes.c_int, []),
+    "dmhm_rank_model_get_params": (ctypes.c_int, [_h, _fp, ctypes.c_int]),
+    "dmhm_rank_model_set_params": (ctypes.c_int, [_h, _fp, ctypes.c_int]),
+    "dmhm_rank_model_gradient": (ctypes.c_int, [_h, _fp, _fp, ctypes.c_int, _fp, _fp]),
+    "dmhm_rank_model_predict": (ctypes.c_int, [_h, _fp, ctypes.c_int, _fp]),
+    "dmhm_rank_model_train": (ctypes.c_int, [_h, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_float, ctypes.c_uint64, _fp]),
+    "dmhm_hmat_rank_features": (ctypes.c_int, [_h, _fp, ctypes.c_int]),
+    "dmhm_last_product_stats": (ctypes.c_int, [_llp]),
+    "dmhm_get_tile_engine": (ctypes.c_int, []),
     "dmhm_hmat_packed_size": (ctypes.c_int, [_h, ctypes.POINTER(ctypes.c_uint64)]),
     "dmhm_hmat_pack": (ctypes.c_int, [_h, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     "dmhm_hmat_unpack": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _hp]),
@@ -148,6 +165,17 @@ def check(code):
 
 
 _initialized = False
+
+
+TILE_ENGINES = {"auto": -1, "lds": 0, "mfma": 1, "mfma_pf": 2, "bf16": 3, "bf16_pf": 4}
+
+
+def set_tile_engine(name):
+    """Force a tile engine ("lds", "mfma", "mfma_pf", "bf16", "bf16_pf") or
+    autotune ("auto", the default); returns the previous setting's name."""
+    prev = load().dmhm_get_tile_engine()
+    check(load().dmhm_set_tile_engine(TILE_ENGINES[name]))
+    return {v: k for k, v in TILE_ENGINES.items()}[prev]
 
 
 def init(device=0):

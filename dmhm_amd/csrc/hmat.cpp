@@ -8,6 +8,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <random>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -55,18 +56,48 @@ struct DeviceBuffer {
 
 struct GeoCtx;
 
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() { HIPCHECK(hipEventCreate(&a)); HIPCHECK(hipEventCreate(&b)); }
+    ~EventPair() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    float ms() { float t = 0; HIPCHECK(hipEventElapsedTime(&t, a, b)); return t; }
+};
+
 struct Library {
     bool ready = false;
     int device = 0;
     hipStream_t stream = nullptr;
     DeviceBuffer ws;            // plan workspace
     DeviceBuffer stageX, stageY;
+    DeviceBuffer featBuf, widthBuf, flagBuf;   // rank-model features, sketch widths, check flags
     double* dScalars = nullptr; // power-iteration scalars
     std::vector<std::weak_ptr<GeoCtx>> geos;
     std::mutex mu;
     int nextLayoutId = 100;
+    // Tile engine of every TILE phase: -1 autotunes per phase (default);
+    // DMHM_TILE_ENGINE=lds|mfma|mfma_pf|bf16|bf16_pf forces one.
+    int tileEngine = -1;
+    bool bf16 = false;          // BF16 operands allowed (set inside the mixed Schulz loop)
+    long untuned = 0;           // tile launches made while tuning (graph capture waits for none)
+    EventPair* tuneEv = nullptr;
+    // Device iteration counter added to every sketch and start-vector seed
+    // while a replayable loop runs (null otherwise), and a pinned word for the
+    // loop's one readback per iteration.
+    const uint64_t* seedOff = nullptr;
+    uint64_t* dCounter = nullptr;
+    double* hReadback = nullptr;
 };
 static Library g_lib;
+
+// Copies on the library stream and waits for them: the stream is created
+// non-blocking, so a plain hipMemcpy (null stream) would not be ordered after
+// work already queued on it (for example the zero fill of a new matrix).
+static hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, g_lib.stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(g_lib.stream);
+}
 
 static void require_ready()
 {
@@ -78,8 +109,10 @@ struct DeviceTables {        // small index tables for per-leaf kernels
     ~DeviceTables() { if (ptr) (void)hipFree(ptr); }
 };
 
+struct OperatorStats;
 struct GeoCtx {
     Geometry geo;
+    std::shared_ptr<OperatorStats> zeroStats;     // for matrices without an operator's statistics
     std::map<std::string, std::unique_ptr<Plan>> plans;
     std::map<int, std::shared_ptr<Layout>> stdLayouts;      // by maxRank
     // diagonal dense leaves (t == s) for AddConstantToDiagonal, per layout id
@@ -121,9 +154,26 @@ static std::shared_ptr<Layout> std_layout(GeoCtx& gc, int maxRank)
 
 using namespace dmhm;
 
+namespace dmhm {
+// Coefficient statistics of an operator for the rank model: per box of the
+// geometry (every level; box (level, id) at levelBase[level] + id), the mean
+// and standard deviation of log|a_ii| over the box's rows, taken from the
+// diagonal of the imported CSR matrix; and, built on first use, the static
+// features of every admissible block on the device.
+struct OperatorStats {
+    std::vector<int> levelBase;
+    std::vector<float> mean, sdev;
+    float globalMean = 0;
+    DeviceBuffer feat;          // nLowRank x kStaticFeatures floats
+    bool featReady = false;
+    ~OperatorStats() { feat.release(); }
+};
+} // namespace dmhm
+
 struct dmhm_hmat_s {
     std::shared_ptr<GeoCtx> gc;
     std::shared_ptr<Layout> lay;
+    std::shared_ptr<OperatorStats> stats;   // of the operator this matrix derives from (or null)
     int prec = DMHM_F64;
     int maxRank = 0;
     void* vals = nullptr;
@@ -140,7 +190,23 @@ struct dmhm_hmat_s {
     const Geometry& geo() const { return gc->geo; }
 };
 
+// Rank model: the perceptron's parameters and Adam state on the device.
+struct dmhm_rank_model_s {
+    int nf = 0, np = 0, step = 0;
+    dmhm::DeviceBuffer params, grads, m, v;
+    ~dmhm_rank_model_s() { params.release(); grads.release(); m.release(); v.release(); }
+};
+
 namespace dmhm {
+
+// Features per admissible block: kStaticFeatures from the geometry and the
+// operator's coefficient statistics, then the previous rank.
+constexpr int kStaticFeatures = 11;
+constexpr int kRankFeatures = kStaticFeatures + 1;
+
+// What the last width-indirect product did (dmhm_last_product_stats).
+struct ProductStats { long long blocks = 0, widthSum = 0, fallbacks = 0, fullWidth = 0; };
+static ProductStats g_lastProduct;
 
 static dmhm_hmat_s* alloc_hmat(std::shared_ptr<GeoCtx> gc, std::shared_ptr<Layout> lay, int prec,
                                int maxRank)
@@ -167,21 +233,71 @@ static Bases bases_for(dmhm_hmat_s* A, dmhm_hmat_s* B, dmhm_hmat_s* C, void* X, 
     return b;
 }
 
+// Tile-engine candidates per execution slot (FP64, FP32, FP32 with BF16
+// operands allowed); FP32 engines stay candidates in the BF16 slot, since a
+// more precise engine that is also faster is the better choice there.
+// (The prefetching MFMA form is not a candidate in FP64: it produced wrong
+// products there on gfx950 while the same source is exact in FP32.)
+static const int kTuneCands[3][5] = {{TILE_LDS, TILE_MFMA, -1, -1, -1},
+                                     {TILE_LDS, TILE_MFMA, TILE_MFMA_PF, -1, -1},
+                                     {TILE_LDS, TILE_MFMA, TILE_MFMA_PF, TILE_MFMA_BF16, TILE_MFMA_BF16_PF}};
+static const int kTuneCount[3] = {2, 3, 5};
+
+// Launches a TILE phase with its engine. Autotuning (no engine forced): the
+// first runs of a phase in a slot time one candidate each (a complete,
+// correct execution every time, synchronised for the timing), after which the
+// fastest candidate is kept for that slot.
+template <typename T>
+static void run_tile_phase(Phase& ph, const Bases& b, T alpha, T beta)
+{
+    hipStream_t s = g_lib.stream;
+    const int slot = sizeof(T) == 8 ? 0 : (g_lib.bf16 ? 2 : 1);
+    if (g_lib.tileEngine >= 0) {
+        int e = g_lib.tileEngine;
+        if ((e == TILE_MFMA_BF16 || e == TILE_MFMA_BF16_PF) && slot != 2) e = e == TILE_MFMA_BF16 ? TILE_MFMA : TILE_MFMA_PF;
+        launch_tile_phase<T>(ph, b, alpha, beta, s, e);
+        return;
+    }
+    Phase::Tune& t = ph.tune[slot];
+    if (t.choice >= 0) {
+        launch_tile_phase<T>(ph, b, alpha, beta, s, t.choice);
+        return;
+    }
+    const int cand = kTuneCands[slot][t.next];
+    EventPair& ev = *g_lib.tuneEv;
+    HIPCHECK(hipEventRecord(ev.a, s));
+    launch_tile_phase<T>(ph, b, alpha, beta, s, cand);
+    HIPCHECK(hipEventRecord(ev.b, s));
+    HIPCHECK(hipEventSynchronize(ev.b));
+    t.ms[t.next] = ev.ms();
+    ++g_lib.untuned;
+    if (++t.next == kTuneCount[slot]) {
+        int best = 0;
+        for (int i = 1; i < kTuneCount[slot]; ++i)
+            if (t.ms[i] < t.ms[best]) best = i;
+        t.choice = kTuneCands[slot][best];
+    }
+}
+
+template <typename T>
+static void run_phase(Phase& ph, const Bases& b, T alpha, T beta, TruncPolicy pol, uint64_t seed)
+{
+    hipStream_t s = g_lib.stream;
+    switch (ph.kind) {
+    case PH_TILE: run_tile_phase<T>(ph, b, alpha, beta); break;
+    case PH_QR: launch_qr_phase<T>(ph, b, s); break;
+    case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
+    case PH_RNG: launch_rng_phase<T>(ph, b, seed, s, g_lib.seedOff); break;
+    case PH_EYE: launch_eye<T>(reinterpret_cast<T*>(b.val[SLOT_WS]) + ph.zeroOff, (int)ph.zeroCount, s); break;
+    case PH_MV_ZERO: case PH_MV_PROJ: case PH_MV_ACC: launch_mv_phase<T>(ph, b, alpha, beta, s); break;
+    default: break;
+    }
+}
+
 template <typename T>
 static void run_plan(Plan& p, const Bases& b, T alpha, T beta, TruncPolicy pol, uint64_t seed)
 {
-    hipStream_t s = g_lib.stream;
-    for (auto& ph : p.phases) {
-        switch (ph.kind) {
-        case PH_TILE: launch_tile_phase<T>(ph, b, alpha, beta, s); break;
-        case PH_QR: launch_qr_phase<T>(ph, b, s); break;
-        case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
-        case PH_RNG: launch_rng_phase<T>(ph, b, seed, s); break;
-        case PH_EYE: launch_eye<T>(reinterpret_cast<T*>(b.val[SLOT_WS]) + ph.zeroOff, (int)ph.zeroCount, s); break;
-        case PH_MV_ZERO: case PH_MV_PROJ: case PH_MV_ACC: launch_mv_phase<T>(ph, b, alpha, beta, s); break;
-        default: break;
-        }
-    }
+    for (auto& ph : p.phases) run_phase<T>(ph, b, alpha, beta, pol, seed);
 }
 
 // Workspace budget (elements) of one H*H batch: 2^33 (64 GiB in FP64) by
@@ -235,30 +351,147 @@ static void multiply_dense(dmhm_hmat_s* h, bool trans, double alpha, const void*
     else run_plan<float>(p, b, (float)alpha, (float)beta, {0, 0}, 0);
 }
 
-static dmhm_hmat_s* multiply_hmat(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, const dmhm_hh_opts& o)
+// Static features of every admissible block of a geometry (kStaticFeatures
+// floats each, on the device, built once per operator statistics): level;
+// log sizes of the target and source boxes; log of their distance in boxes;
+// the signed box offsets per axis over the boxes per axis; mean (relative to
+// the operator's) and spread of log|a_ii| over the target and source boxes.
+static const float* static_features(GeoCtx& gc, std::shared_ptr<OperatorStats> st)
 {
-    if (A->gc != B->gc) throw Error(DMHM_ENONCONF, "Attempted nonconformal matrix-matrix multiply");
-    if (A->prec != B->prec) throw Error(DMHM_EINVAL, "operands must share a precision");
+    const Geometry& g = gc.geo;
+    if (!st) {
+        if (!gc.zeroStats) {
+            gc.zeroStats = std::make_shared<OperatorStats>();
+            int base = 0;
+            for (int l = 0; l < g.numLevels; ++l) {
+                gc.zeroStats->levelBase.push_back(base);
+                base += (int)g.boxes[l].size();
+            }
+            gc.zeroStats->mean.assign(base, 0.0f);
+            gc.zeroStats->sdev.assign(base, 0.0f);
+        }
+        st = gc.zeroStats;
+    }
+    if (st->featReady) return static_cast<const float*>(st->feat.ptr);
+    const int nl = (int)g.lrBlocks.size();
+    std::vector<float> f((size_t)std::max(nl, 1) * kStaticFeatures, 0.0f);
+    const double logn = std::log2((double)std::max(2, g.n));
+    for (int i = 0; i < nl; ++i) {
+        const Block& B = g.blocks[g.lrBlocks[i]];
+        const Box& tb = g.boxes[B.level][B.t];
+        const Box& sb = g.boxes[B.level][B.s];
+        float* x = f.data() + (size_t)i * kStaticFeatures;
+        int dist = 1;
+        for (int a = 0; a < g.dim; ++a) dist = std::max(dist, std::abs(tb.coord[a] - sb.coord[a]));
+        const double nbox = (double)(1 << B.level);
+        x[0] = (float)B.level / (float)std::max(1, g.numLevels - 1);
+        x[1] = (float)(std::log2((double)std::max(1, tb.size)) / logn);
+        x[2] = (float)(std::log2((double)std::max(1, sb.size)) / logn);
+        x[3] = (float)(std::log2((double)dist) / std::max(1, B.level));
+        x[4] = (float)((tb.coord[0] - sb.coord[0]) / nbox);
+        x[5] = (float)((tb.coord[1] - sb.coord[1]) / nbox);
+        x[6] = g.dim == 3 ? (float)((tb.coord[2] - sb.coord[2]) / nbox) : 0.0f;
+        const int it = st->levelBase[B.level] + B.t, is = st->levelBase[B.level] + B.s;
+        x[7] = st->mean[it] - st->globalMean;
+        x[8] = st->sdev[it];
+        x[9] = st->mean[is] - st->globalMean;
+        x[10] = st->sdev[is];
+    }
+    st->feat.ensure(f.size() * sizeof(float));
+    HIPCHECK(copy_sync(st->feat.ptr, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    st->featReady = true;
+    return static_cast<const float*>(st->feat.ptr);
+}
+
+// Features of the blocks of a product with left operand A into `out` (device,
+// nLowRank x kRankFeatures): the static features of A's operator and the rank
+// of each block in A (the "previous rank": in a Schulz step the left operand
+// is the current iterate, or Z = I - X A).
+static void rank_features(dmhm_hmat_s* A, float* out)
+{
+    const float* st = static_features(*A->gc, A->stats);
+    launch_rank_features(st, kStaticFeatures, A->ranks, 1.0f / std::max(1, A->maxRank), out,
+                         (int)A->geo().lrBlocks.size(), g_lib.stream);
+}
+
+// C := alpha A B into an existing C of the same geometry and precision (its
+// layout is the plan's output layout). The plan writes every dense entry, every
+// U/V column and every rank of C, so C's previous contents do not matter.
+// With a rank model in tolerance mode the sketches are width-indirect: the
+// model predicts every block's rank from its features, the sketch takes that
+// plus the oversampling (at most maxRank + oversample columns), and blocks
+// whose SVD flags the sketch as too narrow are recomputed at full width.
+static void multiply_into(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, dmhm_hmat_s* C, const dmhm_hh_opts& o)
+{
+    if (A->gc != B->gc || A->gc != C->gc) throw Error(DMHM_ENONCONF, "Attempted nonconformal matrix-matrix multiply");
+    if (A->prec != B->prec || A->prec != C->prec) throw Error(DMHM_EINVAL, "operands must share a precision");
     GeoCtx& gc = *A->gc;
-    const int maxRank = A->maxRank;
-    auto LC = std_layout(gc, maxRank);
-    std::unique_ptr<dmhm_hmat_s> C(alloc_hmat(A->gc, LC, A->prec, maxRank));
+    const int maxRank = C->maxRank;
+    const Layout& LC = *C->lay;
     const int over = o.oversample > 0 ? o.oversample : 4;
+    dmhm_rank_model_s* rm = o.rankModel;
+    const bool learned = rm != nullptr && o.tolMode == 1;
     const std::string key = "hh:" + std::to_string(A->lay->id) + ":" + std::to_string(B->lay->id) + ":" +
-                            std::to_string(LC->id) + ":" + std::to_string(maxRank) + ":" +
-                            std::to_string(over) + ":" + std::to_string(A->full) + std::to_string(B->full);
+                            std::to_string(LC.id) + ":" + std::to_string(maxRank) + ":" +
+                            std::to_string(over) + ":" + std::to_string(A->full) + std::to_string(B->full) +
+                            (learned ? ":w" : "");
     Plan& p = get_plan(gc, key, [&] {
         Planner pl(gc.geo);
         pl.setWorkspaceBudget(hh_batch_budget());
         pl.setStacking(A->full, B->full);
-        return pl.hhProduct(*A->lay, *B->lay, *LC, maxRank, over);
+        pl.setWidthIndirect(learned);
+        return pl.hhProduct(*A->lay, *B->lay, LC, maxRank, over);
     });
     ensure_ws(p, A->esize());
-    Bases b = bases_for(A, B, C.get(), nullptr, nullptr);
+    Bases b = bases_for(A, B, C, nullptr, nullptr);
     TruncPolicy pol{o.tolMode, o.relTol};
-    if (A->prec == DMHM_F64) run_plan<double>(p, b, alpha, 0.0, pol, o.seed);
-    else run_plan<float>(p, b, (float)alpha, 0.0f, pol, o.seed);
+    auto run = [&] {
+        if (A->prec == DMHM_F64) run_plan<double>(p, b, alpha, 0.0, pol, o.seed);
+        else run_plan<float>(p, b, (float)alpha, 0.0f, pol, o.seed);
+    };
+    if (!learned) {
+        run();
+        C->full = true;
+        return;
+    }
+    const int nl = LC.nLowRank();
+    const int kS = maxRank + over;
+    g_lib.featBuf.ensure((size_t)std::max(nl, 1) * kRankFeatures * sizeof(float));
+    g_lib.widthBuf.ensure((size_t)std::max(nl, 1) * sizeof(int));
+    g_lib.flagBuf.ensure((size_t)std::max(nl, 1) * sizeof(int));
+    float* feat = static_cast<float*>(g_lib.featBuf.ptr);
+    int* widths = static_cast<int*>(g_lib.widthBuf.ptr);
+    int* flags = static_cast<int*>(g_lib.flagBuf.ptr);
+    rank_features(A, feat);
+    launch_mlp_infer(static_cast<const float*>(rm->params.ptr), rm->nf, feat, nl, nullptr, widths,
+                     (float)o.rankMargin, over, over + 1, kS, g_lib.stream);
+    launch_fill_int(flags, nl, 0, g_lib.stream);
+    b.rank[SLOT_WIDTH] = widths;
+    b.rank[SLOT_FLAG] = flags;
+    run();
+    // A-posteriori check: widen the flagged blocks to the full sketch and
+    // recompute (the other blocks come out identical: same widths and seeds).
+    int* cnt = reinterpret_cast<int*>(g_lib.dScalars + 4);
+    long long* wsum = reinterpret_cast<long long*>(g_lib.dScalars + 5);
+    launch_zero_double(g_lib.dScalars + 4, 2, g_lib.stream);
+    launch_widen_flagged(widths, flags, nl, kS, cnt, wsum, g_lib.stream);
+    struct { int cnt; int pad; long long wsum; } host{};
+    HIPCHECK(copy_sync(&host, g_lib.dScalars + 4, 16, hipMemcpyDeviceToHost));
+    if (host.cnt > 0) run();
+    g_lastProduct.blocks = nl;
+    g_lastProduct.widthSum = host.wsum;
+    g_lastProduct.fallbacks = host.cnt;
+    g_lastProduct.fullWidth = (long long)nl * kS;
     C->full = true;
+}
+
+static dmhm_hmat_s* multiply_hmat(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, const dmhm_hh_opts& o)
+{
+    if (A->gc != B->gc) throw Error(DMHM_ENONCONF, "Attempted nonconformal matrix-matrix multiply");
+    if (A->prec != B->prec) throw Error(DMHM_EINVAL, "operands must share a precision");
+    std::unique_ptr<dmhm_hmat_s> C(alloc_hmat(A->gc, std_layout(*A->gc, A->maxRank), A->prec, A->maxRank));
+    C->stats = A->stats ? A->stats : B->stats;
+    multiply_into(alpha, A, B, C.get(), o);
     return C.release();
 }
 
@@ -400,13 +633,13 @@ static void rounded_update_dev(int maxRank, double alpha, int m, int n, int rA, 
         for (int i = 0; i < n; ++i) host[Vc + i + (size_t)(rA + j) * np] = BV[i + (size_t)j * n];
     }
     job.ws.ensure(o * sizeof(double));
-    HIPCHECK(hipMemcpy(job.ws.ptr, host.data(), o * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(copy_sync(job.ws.ptr, host.data(), o * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMalloc((void**)&job.rank, sizeof(int)));
     job.plan.phases.resize(5);
     Phase& q = job.plan.phases[0];
     q.kind = PH_QR;
-    q.qr.push_back(QrTask{Uc, R1, mp, r, mp, r, SLOT_WS, 0, 0, 0, 0});
-    q.qr.push_back(QrTask{Vc, R2, np, r, np, r, SLOT_WS, 0, 0, 0, 0});
+    q.qr.push_back(QrTask{Uc, R1, mp, r, mp, r, SLOT_WS});
+    q.qr.push_back(QrTask{Vc, R2, np, r, np, r, SLOT_WS});
     job.plan.phases[1].kind = PH_TILE;                        // W = R1 R2^T
     add_gemm_tiles(job.plan.phases[1], Wm, r, r, R1, r, r, R2, r, CF_TRANS_R);
     Phase& sv = job.plan.phases[2];
@@ -418,8 +651,8 @@ static void rounded_update_dev(int maxRank, double alpha, int m, int n, int rA, 
     add_gemm_tiles(job.plan.phases[4], oV, np, r, Vc, np, r, G1, r, 0);
     job.run();
     int keep = 0;
-    HIPCHECK(hipMemcpy(&keep, job.rank, sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(host.data(), job.ws.ptr, o * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(copy_sync(&keep, job.rank, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(copy_sync(host.data(), job.ws.ptr, o * sizeof(double), hipMemcpyDeviceToHost));
     for (int j = 0; j < keep; ++j) {
         for (int i = 0; i < m; ++i) outU[i + (size_t)j * m] = host[oU + i + (size_t)j * mp];
         for (int i = 0; i < n; ++i) outV[i + (size_t)j * n] = host[oV + i + (size_t)j * np];
@@ -449,19 +682,19 @@ static void compress_dense_dev(int maxRank, int m, int n, const double* D, doubl
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < m; ++i) host[Dd + i + (size_t)j * mp] = D[i + (size_t)j * m];
     job.ws.ensure(o * sizeof(double));
-    HIPCHECK(hipMemcpy(job.ws.ptr, host.data(), o * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(copy_sync(job.ws.ptr, host.data(), o * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMalloc((void**)&job.rank, sizeof(int)));
     job.plan.phases.resize(3);
     job.plan.phases[0].kind = PH_QR;
-    job.plan.phases[0].qr.push_back(QrTask{Dd, R, mp, k, mp, k, SLOT_WS, 0, 0, 0, 0});
+    job.plan.phases[0].qr.push_back(QrTask{Dd, R, mp, k, mp, k, SLOT_WS});
     job.plan.phases[1].kind = PH_SVD;                         // R = U S V^T: G1 = V S, G2 = U
     job.plan.phases[1].svd.push_back(SvdTask{R, G1, G2, k, k, 0, cap});
     job.plan.phases[2].kind = PH_TILE;                        // out U = Q U_R
     add_gemm_tiles(job.plan.phases[2], oU, mp, k, Dd, mp, k, G2, k, 0);
     job.run();
     int keep = 0;
-    HIPCHECK(hipMemcpy(&keep, job.rank, sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(host.data(), job.ws.ptr, o * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(copy_sync(&keep, job.rank, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(copy_sync(host.data(), job.ws.ptr, o * sizeof(double), hipMemcpyDeviceToHost));
     for (int j = 0; j < keep; ++j) {
         for (int i = 0; i < m; ++i) outU[i + (size_t)j * m] = host[oU + i + (size_t)j * mp];
         for (int i = 0; i < n; ++i) outV[i + (size_t)j * n] = host[G1 + i + (size_t)j * k];
@@ -503,13 +736,13 @@ static dmhm_hmat_s* create_from_lowrank(int dim, const int* sizes, int L, int ma
             for (int rr = 0; rr < sb.size; ++rr) host[lay->vOff[i] + rr + (size_t)p * sb.size] = V[(sb.off + rr) + (size_t)p * n];
         }
     }
-    if (prec == DMHM_F64) HIPCHECK(hipMemcpy(h->vals, host.data(), host.size() * 8, hipMemcpyHostToDevice));
+    if (prec == DMHM_F64) HIPCHECK(copy_sync(h->vals, host.data(), host.size() * 8, hipMemcpyHostToDevice));
     else {
         std::vector<float> hf(host.begin(), host.end());
-        HIPCHECK(hipMemcpy(h->vals, hf.data(), hf.size() * 4, hipMemcpyHostToDevice));
+        HIPCHECK(copy_sync(h->vals, hf.data(), hf.size() * 4, hipMemcpyHostToDevice));
     }
     std::vector<int> ranks(g.lrBlocks.size(), r);
-    if (!ranks.empty()) HIPCHECK(hipMemcpy(h->ranks, ranks.data(), ranks.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!ranks.empty()) HIPCHECK(copy_sync(h->ranks, ranks.data(), ranks.size() * sizeof(int), hipMemcpyHostToDevice));
     h->full = r >= std::max(maxRank, 1);
     return h.release();
 }
@@ -524,6 +757,7 @@ static dmhm_hmat_s* copy_hmat(dmhm_hmat_s* s, int prec)
     HIPCHECK(hipMemcpyAsync(d->ranks, s->ranks, s->lay->nLowRank() * sizeof(int), hipMemcpyDeviceToDevice,
                             g_lib.stream));
     d->full = s->full;
+    d->stats = s->stats;
     return d.release();
 }
 
@@ -531,7 +765,7 @@ template <typename T>
 static void upload(std::vector<T>& v, T** d)
 {
     HIPCHECK(hipMalloc((void**)d, std::max<size_t>(v.size(), 1) * sizeof(T)));
-    if (!v.empty()) HIPCHECK(hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) HIPCHECK(copy_sync(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 
 static dmhm_hmat_s* transpose_hmat(dmhm_hmat_s* s)
@@ -553,6 +787,7 @@ static dmhm_hmat_s* transpose_hmat(dmhm_hmat_s* s)
     }
     std::unique_ptr<dmhm_hmat_s> d(alloc_hmat(s->gc, Ld, s->prec, s->maxRank));
     d->full = s->full;
+    d->stats = s->stats;
     std::vector<int64_t> so, dof; std::vector<int> mm, nn;
     for (size_t i = 0; i < g.denseBlocks.size(); ++i) {
         const int b = g.denseBlocks[i];
@@ -664,19 +899,22 @@ static void set_to_random(dmhm_hmat_s* h, uint64_t seed)
     launch_fill_int(h->ranks, L.nLowRank(), h->maxRank, st);
 }
 
+// Power iteration of EstimateTwoNorm (hmat_tools.hpp:1504-1544) as launches
+// only: the log of the estimate accumulates in dScalars[1] on the device, so
+// the sequence can be captured in a HIP graph; read_two_norm fetches it.
 template <typename T>
-static double estimate_two_norm_t(dmhm_hmat_s* h, double theta, double confidence, uint64_t seed)
+static void launch_two_norm_t(dmhm_hmat_s* h, double confidence, double theta, uint64_t seed)
 {
     const Geometry& g = h->geo();
     const int n = g.n;
     const int k = (int)std::ceil(std::log(0.8 * std::sqrt((double)n) * std::pow(10.0, confidence)) /
                                  std::log(theta));
-    g_lib.stageX.ensure(2 * (size_t)n * sizeof(T) + 64);
+    g_lib.stageX.ensure(2 * (size_t)n * sizeof(double) + 64);
     T* x = (T*)g_lib.stageX.ptr;
     T* y = x + n;
     hipStream_t st = g_lib.stream;
     double* d = g_lib.dScalars;       // d[0] sumsq, d[1] log accumulator
-    launch_gaussian<T>(x, n, seed, 7, st);
+    launch_gaussian<T>(x, n, seed, 7, st, g_lib.seedOff);
     launch_zero_double(d, 2, st);
     launch_sumsq<T>(x, n, d, st);
     launch_normalize<T>(x, n, d, nullptr, 0.0, st);
@@ -687,20 +925,91 @@ static double estimate_two_norm_t(dmhm_hmat_s* h, double theta, double confidenc
         launch_normalize<T>(y, n, d, d + 1, 1.0 / k, st);
         std::swap(x, y);
     }
-    double acc = 0;
-    HIPCHECK(hipMemcpyAsync(&acc, d + 1, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    return theta * std::exp(acc);
+}
+
+static void launch_two_norm(dmhm_hmat_s* h, double theta, double confidence, uint64_t seed)
+{
+    if (h->prec == DMHM_F64) launch_two_norm_t<double>(h, confidence, theta, seed);
+    else launch_two_norm_t<float>(h, confidence, theta, seed);
+}
+
+static double read_two_norm(double theta)
+{
+    HIPCHECK(hipMemcpyAsync(g_lib.hReadback, g_lib.dScalars + 1, sizeof(double), hipMemcpyDeviceToHost,
+                            g_lib.stream));
+    HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    return theta * std::exp(g_lib.hReadback[0]);
 }
 
 static double estimate_two_norm(dmhm_hmat_s* h, double theta, double confidence, uint64_t seed)
 {
     if (theta <= 1) throw Error(DMHM_EINVAL, "Theta must be > 1");
     if (confidence <= 0) throw Error(DMHM_EINVAL, "Confidence must be positive");
-    return h->prec == DMHM_F64 ? estimate_two_norm_t<double>(h, theta, confidence, seed)
-                               : estimate_two_norm_t<float>(h, theta, confidence, seed);
+    launch_two_norm(h, theta, confidence, seed);
+    return read_two_norm(theta);
 }
 
+// Makes the device iteration counter offset every seed for its lifetime.
+struct SeedOffsetScope {
+    SeedOffsetScope() { g_lib.seedOff = g_lib.dCounter; }
+    ~SeedOffsetScope() { g_lib.seedOff = nullptr; }
+};
+
+// One Schulz step X_out = (2I - X_in A) X_in (Invert-incl.hpp:160-192) as
+// launches only: the counter bump, Z = -X_in A, Z += I, the residual power
+// iteration on Z (stop mode), Z += I, X_out = Z X_in. Every seed is the
+// step's fixed base plus the device counter, so a captured step replayed by a
+// HIP graph launches exactly what the eager step launches.
+struct SchulzStep {
+    dmhm_hmat_s *A, *Xin, *Xout, *Z;
+    bool stop;
+    double theta, confidence;
+    dmhm_hh_opts hh;
+    uint64_t seed;
+    void launch() const {
+        launch_bump(g_lib.dCounter, g_lib.stream);
+        dmhm_hh_opts o = hh;
+        o.seed = seed;
+        multiply_into(-1.0, Xin, A, Z, o);
+        add_diag(Z, 1.0);
+        if (stop) launch_two_norm(Z, theta, confidence, seed + 1);
+        add_diag(Z, 1.0);
+        o.seed = seed + 2;
+        multiply_into(1.0, Z, Xin, Xout, o);
+    }
+};
+
+// A captured step and its executable graph (one per direction of the
+// ping-pong between the two iterate buffers).
+struct StepGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~StepGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    void capture(const SchulzStep& st) {
+        HIPCHECK(hipStreamBeginCapture(g_lib.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            st.launch();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(g_lib.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(g_lib.stream, &graph));
+        HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    void replay() { HIPCHECK(hipGraphLaunch(exec, g_lib.stream)); }
+};
+
+// SchulzInvert (Invert-incl.hpp:128-196): X_0 = alpha A^T, alpha = 2/est^2,
+// X <- (2I - X A) X. The iterates live in two preallocated buffers of the
+// standard layout (plus Z and the transposed start), so the loop allocates
+// nothing after its first two steps; with o.graph set, every later step of a
+// precision is one HIP graph replay (two graphs, one per ping-pong
+// direction) followed by the one readback of the residual estimate.
 static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report* rep)
 {
     const auto t0 = std::chrono::steady_clock::now();
@@ -709,65 +1018,128 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
     bool stop = false;
     int numIt = o.numIterations;
     if (numIt <= 0) { numIt = o.maxIterations > 0 ? o.maxIterations : 30; stop = true; }
+    if (numIt > 4096) throw Error(DMHM_EINVAL, "at most 4096 Schulz iterations");
     uint64_t seed = o.hh.seed;
-    dmhm_hh_opts hh = o.hh;
-    std::unique_ptr<dmhm_hmat_s> Alow;
-    dmhm_hmat_s* Acur = A;
-    bool low = false;
-    if (o.mixed && A->prec == DMHM_F64) {
-        Alow.reset(copy_hmat(A, DMHM_F32));
-        Acur = Alow.get();
-        low = true;
-    }
-    const double est = estimate_two_norm(Acur, o.theta, o.confidence, seed++);
+    GeoCtx& gc = *A->gc;
+    const int maxRank = A->maxRank;
+    auto LS = std_layout(gc, maxRank);
+
+    // Operator and buffers per precision (index 0: FP64, 1: FP32).
+    struct PrecState {
+        dmhm_hmat_s* A = nullptr;
+        std::unique_ptr<dmhm_hmat_s> Aown, X[2], Z;
+        StepGraph g[2][2];                  // [BF16 operands][ping-pong direction]
+        bool captured[2][2] = {{false, false}, {false, false}};
+        bool tuned[2] = {false, false};     // an eager steady step ran with every phase tuned
+    } ps[2];
+    int cur = (o.mixed && A->prec == DMHM_F64) ? 1 : (A->prec == DMHM_F64 ? 0 : 1);
+    const int target = A->prec == DMHM_F64 ? 0 : 1;    // precision of the result
+    auto prepare = [&](int pi) {
+        PrecState& P = ps[pi];
+        if (P.A) return;
+        const int prec = pi == 0 ? DMHM_F64 : DMHM_F32;
+        if (A->prec == prec) P.A = A;
+        else { P.Aown.reset(copy_hmat(A, prec)); P.A = P.Aown.get(); }
+        for (auto& x : P.X) x.reset(alloc_hmat(A->gc, LS, prec, maxRank));
+        P.Z.reset(alloc_hmat(A->gc, LS, prec, maxRank));
+    };
+    prepare(cur);
+    launch_zero_double(reinterpret_cast<double*>(g_lib.dCounter), 1, g_lib.stream);   // counter = 0
+    SeedOffsetScope seeds;
+    const double est = estimate_two_norm(ps[cur].A, o.theta, o.confidence, seed);
     const double alpha = 2.0 / (est * est);
-    std::unique_ptr<dmhm_hmat_s> X(transpose_hmat(Acur));
-    scale_hmat(X.get(), alpha);
-    int it = 0, itLow = 0;
+    std::unique_ptr<dmhm_hmat_s> X0(transpose_hmat(ps[cur].A));
+    scale_hmat(X0.get(), alpha);
+
+    int it = 0, itLow = 0, itGraph = 0, itBf16 = 0;
     double e = -1;
     std::vector<double> hist;
     auto stalled = [&]() {
         const size_t n = hist.size();
         return o.stallStop && n >= 3 && hist[n - 1] < 0.1 && hist[n - 1] > 0.5 * hist[n - 3];
     };
+    dmhm_hmat_s* Xc = X0.get();       // current iterate
+    int side = 0;                     // X[side] receives the next iterate
+    int stepsInPrec = 0;
+    struct Bf16Scope {              // BF16 operands allowed in the tile phases while alive
+        explicit Bf16Scope(bool on) { g_lib.bf16 = on; }
+        ~Bf16Scope() { g_lib.bf16 = false; }
+    };
     for (int kk = 0; kk < numIt; ++kk) {
         it = kk + 1;
+        const bool low = cur != target;
         if (low) ++itLow;
-        hh.seed = seed++;
-        std::unique_ptr<dmhm_hmat_s> Z(multiply_hmat(-1.0, X.get(), Acur, hh));
-        add_diag(Z.get(), 1.0);
+        PrecState& P = ps[cur];
+        // BF16 operands (FP32 accumulation) for the early low-precision steps:
+        // until the residual estimate drops below bf16Tol.
+        const int bf = (low && cur == 1 && o.bf16 && (hist.empty() || std::fabs(e) >= o.bf16Tol)) ? 1 : 0;
+        if (bf) ++itBf16;
+        Bf16Scope bscope(bf != 0);
+        SchulzStep st{P.A, Xc, P.X[side].get(), P.Z.get(), stop, o.theta, o.confidence, o.hh,
+                      seed + 1 + 3 * (uint64_t)cur};
+        // Eager steps build the plans, size the workspace and autotune the
+        // tile engines (the very first step reads X_0, whose layout and rank
+        // profile differ); once a steady step of this precision and operand
+        // mode ran with every phase tuned, later steps replay captured graphs.
+        const bool steady = Xc == P.X[0].get() || Xc == P.X[1].get();
+        if (o.graph && steady && stepsInPrec >= 2 && P.tuned[bf]) {
+            if (!P.captured[bf][side]) { P.g[bf][side].capture(st); P.captured[bf][side] = true; }
+            P.g[bf][side].replay();
+            ++itGraph;
+        } else {
+            const long u0 = g_lib.untuned;
+            st.launch();
+            if (steady && g_lib.untuned == u0) P.tuned[bf] = true;
+        }
+        ++stepsInPrec;
+        Xc = P.X[side].get();
+        side ^= 1;
         bool done = false;
         if (stop) {
-            e = estimate_two_norm(Z.get(), o.theta, o.confidence, seed++);
+            e = read_two_norm(o.theta);
             if (rep && kk < 128) rep->history[kk] = e;
             hist.push_back(e);
             if (!low && (std::fabs(e) < o.stopTol || stalled())) done = true;
         }
-        add_diag(Z.get(), 1.0);
-        hh.seed = seed++;
-        std::unique_ptr<dmhm_hmat_s> Xn(multiply_hmat(1.0, Z.get(), X.get(), hh));
-        X = std::move(Xn);
         if (done) break;
         if (low && stop && (std::fabs(e) < o.switchTol || stalled())) {
-            // Switch to FP64 for the remaining iterations.
-            X.reset(copy_hmat(X.get(), DMHM_F64));
-            Acur = A;
-            low = false;
+            // Finish in the result's precision: convert the iterate into its buffers.
+            prepare(target);
+            PrecState& T = ps[target];
+            launch_convert<float, double>((const float*)Xc->vals, (double*)T.X[0]->vals, LS->totalElems(),
+                                          g_lib.stream);
+            HIPCHECK(hipMemcpyAsync(T.X[0]->ranks, Xc->ranks, LS->nLowRank() * sizeof(int), hipMemcpyDeviceToDevice,
+                                    g_lib.stream));
+            T.X[0]->full = Xc->full;
+            Xc = T.X[0].get();
+            side = 1;
+            cur = target;
+            stepsInPrec = 0;
             hist.clear();
         }
     }
-    if (low) { X.reset(copy_hmat(X.get(), DMHM_F64)); }
+    // The result in A's precision.
+    std::unique_ptr<dmhm_hmat_s> Xfinal;
+    if (Xc->prec != A->prec) {
+        prepare(target);
+        Xfinal.reset(copy_hmat(Xc, A->prec));
+        Xc = Xfinal.get();
+    }
     HIPCHECK(hipStreamSynchronize(g_lib.stream));
-    // Replace A's contents with X (CopyFrom, Invert-incl.hpp:195).
-    A->full = X->full;
-    std::swap(A->lay, X->lay);
-    std::swap(A->vals, X->vals);
-    std::swap(A->ranks, X->ranks);
+    // Replace A's contents with X (CopyFrom, Invert-incl.hpp:195); A's old
+    // storage is released with the buffer that held X.
+    dmhm_hmat_s* holder = Xc;
+    A->full = holder->full;
+    std::swap(A->lay, holder->lay);
+    std::swap(A->vals, holder->vals);
+    std::swap(A->ranks, holder->ranks);
     if (rep) {
         rep->iterations = it;
         rep->iterationsLowPrecision = itLow;
         rep->residualEstimate = e;
         rep->normEstimate = est;
+        rep->graphIterations = itGraph;
+        rep->iterationsBf16 = itBf16;
         rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     if (stop && !(std::fabs(e) < o.stopTol))
@@ -783,8 +1155,8 @@ static void to_dense_t(dmhm_hmat_s* h, double* out, int ld)
     std::vector<T> v(L.totalElems());
     std::vector<int> r(L.nLowRank());
     HIPCHECK(hipStreamSynchronize(g_lib.stream));
-    HIPCHECK(hipMemcpy(v.data(), h->vals, v.size() * sizeof(T), hipMemcpyDeviceToHost));
-    if (!r.empty()) HIPCHECK(hipMemcpy(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(copy_sync(v.data(), h->vals, v.size() * sizeof(T), hipMemcpyDeviceToHost));
+    if (!r.empty()) HIPCHECK(copy_sync(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
     for (int j = 0; j < g.n; ++j)
         for (int i = 0; i < g.n; ++i) out[i + (size_t)j * ld] = 0.0;
     for (size_t i = 0; i < g.denseBlocks.size(); ++i) {
@@ -807,6 +1179,37 @@ static void to_dense_t(dmhm_hmat_s* h, double* out, int ld)
                 out[(tb.off + rr) + (size_t)(sb.off + c) * ld] = s;
             }
     }
+}
+
+// Per-box mean and standard deviation of log|a_ii| (the diagonal carries the
+// local coefficient of a stencil operator: the sum of its face coefficients).
+static std::shared_ptr<OperatorStats> operator_stats(const Geometry& g, const int* rowptr, const int* colind,
+                                                     const double* vals)
+{
+    auto st = std::make_shared<OperatorStats>();
+    std::vector<double> ld(g.n, 0.0);
+    double tot = 0;
+    for (int i = 0; i < g.n; ++i) {
+        double d = 0;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
+            if (colind[k] == i) d = vals[k];
+        ld[i] = std::log(std::max(std::fabs(d), 1e-300));
+        tot += ld[i];
+    }
+    st->globalMean = (float)(tot / std::max(1, g.n));
+    int base = 0;
+    for (int l = 0; l < g.numLevels; ++l) {
+        st->levelBase.push_back(base);
+        for (const Box& B : g.boxes[l]) {
+            double s1 = 0, s2 = 0;
+            for (int i = B.off; i < B.off + B.size; ++i) { s1 += ld[i]; s2 += ld[i] * ld[i]; }
+            const double m = s1 / std::max(1, B.size);
+            st->mean.push_back((float)m);
+            st->sdev.push_back((float)std::sqrt(std::max(0.0, s2 / std::max(1, B.size) - m * m)));
+        }
+        base += (int)g.boxes[l].size();
+    }
+    return st;
 }
 
 static dmhm_hmat_s* create_from_csr(int dim, const int* sizes, int L, int maxRank, bool strong, int prec,
@@ -880,13 +1283,14 @@ static dmhm_hmat_s* create_from_csr(int dim, const int* sizes, int L, int maxRan
             }
         }
     if (prec == DMHM_F64) {
-        HIPCHECK(hipMemcpy(h->vals, host.data(), hostElems * 8, hipMemcpyHostToDevice));
+        HIPCHECK(copy_sync(h->vals, host.data(), hostElems * 8, hipMemcpyHostToDevice));
     } else {
         std::vector<float> hf(host.begin(), host.end());
-        HIPCHECK(hipMemcpy(h->vals, hf.data(), hostElems * 4, hipMemcpyHostToDevice));
+        HIPCHECK(copy_sync(h->vals, hf.data(), hostElems * 4, hipMemcpyHostToDevice));
     }
     if (!importRank.empty())
-        HIPCHECK(hipMemcpy(h->ranks, importRank.data(), importRank.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHECK(copy_sync(h->ranks, importRank.data(), importRank.size() * sizeof(int), hipMemcpyHostToDevice));
+    h->stats = operator_stats(g, rowptr, colind, vals);
     return h.release();
 }
 
@@ -896,7 +1300,7 @@ static void matvec_bytes(dmhm_hmat_s* h, int k, double* bytes, double* flops)
     const Layout& L = *h->lay;
     std::vector<int> r(L.nLowRank());
     HIPCHECK(hipStreamSynchronize(g_lib.stream));
-    if (!r.empty()) HIPCHECK(hipMemcpy(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (!r.empty()) HIPCHECK(copy_sync(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
     double entries = 0;
     for (int b : g.denseBlocks) entries += (double)g.tbox(b).size * g.sbox(b).size;
     for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
@@ -908,27 +1312,7 @@ static void matvec_bytes(dmhm_hmat_s* h, int k, double* bytes, double* flops)
     if (flops) *flops = 2.0 * k * entries;
 }
 
-template <typename T>
-static void run_phase(Phase& ph, const Bases& b, T alpha, T beta, TruncPolicy pol, uint64_t seed)
-{
-    hipStream_t s = g_lib.stream;
-    switch (ph.kind) {
-    case PH_TILE: launch_tile_phase<T>(ph, b, alpha, beta, s); break;
-    case PH_QR: launch_qr_phase<T>(ph, b, s); break;
-    case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
-    case PH_RNG: launch_rng_phase<T>(ph, b, seed, s); break;
-    case PH_EYE: launch_eye<T>(reinterpret_cast<T*>(b.val[SLOT_WS]) + ph.zeroOff, (int)ph.zeroCount, s); break;
-    case PH_MV_ZERO: case PH_MV_PROJ: case PH_MV_ACC: launch_mv_phase<T>(ph, b, alpha, beta, s); break;
-    default: break;
-    }
-}
 
-struct EventPair {
-    hipEvent_t a = nullptr, b = nullptr;
-    EventPair() { HIPCHECK(hipEventCreate(&a)); HIPCHECK(hipEventCreate(&b)); }
-    ~EventPair() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
-    float ms() { float t = 0; HIPCHECK(hipEventElapsedTime(&t, a, b)); return t; }
-};
 
 static void profile_matvec(dmhm_hmat_s* h, int k, int reps, double* ms, double* bytes)
 {
@@ -974,7 +1358,7 @@ static void profile_matvec(dmhm_hmat_s* h, int k, int reps, double* ms, double* 
     ms[0] = tot / reps; ms[1] = t0 / reps; ms[2] = t1 / reps;
     // Algorithmic bytes at the current ranks.
     std::vector<int> r(L.nLowRank());
-    if (!r.empty()) HIPCHECK(hipMemcpy(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (!r.empty()) HIPCHECK(copy_sync(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
     double vEnt = 0, uEnt = 0, dEnt = 0;
     for (int bd : g.denseBlocks) dEnt += (double)g.tbox(bd).size * g.sbox(bd).size;
     for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
@@ -1082,6 +1466,18 @@ int dmhm_init(int device)
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
         HIPCHECK(hipMalloc((void**)&g_lib.dScalars, 16 * sizeof(double)));
+        HIPCHECK(hipMalloc((void**)&g_lib.dCounter, sizeof(uint64_t)));
+        HIPCHECK(hipHostMalloc((void**)&g_lib.hReadback, 4 * sizeof(double), hipHostMallocDefault));
+        if (const char* e = getenv("DMHM_TILE_ENGINE")) {
+            const std::string v(e);
+            if (v == "lds") g_lib.tileEngine = TILE_LDS;
+            else if (v == "mfma") g_lib.tileEngine = TILE_MFMA;
+            else if (v == "mfma_pf") g_lib.tileEngine = TILE_MFMA_PF;
+            else if (v == "bf16") g_lib.tileEngine = TILE_MFMA_BF16;
+            else if (v == "bf16_pf") g_lib.tileEngine = TILE_MFMA_BF16_PF;
+            else if (v != "auto") throw Error(DMHM_EINVAL, "DMHM_TILE_ENGINE must be auto, lds, mfma, mfma_pf, bf16 or bf16_pf");
+        }
+        if (!g_lib.tuneEv) g_lib.tuneEv = new EventPair();
         g_lib.device = device;
         g_lib.ready = true;
     });
@@ -1092,8 +1488,13 @@ int dmhm_finalize(void)
     return guard([&] {
         if (!g_lib.ready) return;
         HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        delete g_lib.tuneEv;
+        g_lib.tuneEv = nullptr;
         g_lib.ws.release(); g_lib.stageX.release(); g_lib.stageY.release();
+        g_lib.featBuf.release(); g_lib.widthBuf.release(); g_lib.flagBuf.release();
         (void)hipFree(g_lib.dScalars);
+        (void)hipFree(g_lib.dCounter);
+        (void)hipHostFree(g_lib.hReadback);
         (void)hipStreamDestroy(g_lib.stream);
         g_lib.ready = false;
     });
@@ -1211,7 +1612,7 @@ int dmhm_hmat_info(dmhm_hmat h, dmhm_info* info)
         info->storageBytes = (long long)(h->lay->totalElems() * h->esize());
         std::vector<int> r(h->lay->nLowRank());
         HIPCHECK(hipStreamSynchronize(g_lib.stream));
-        if (!r.empty()) HIPCHECK(hipMemcpy(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
+        if (!r.empty()) HIPCHECK(copy_sync(r.data(), h->ranks, r.size() * sizeof(int), hipMemcpyDeviceToHost));
         long long s = 0; int mx = 0;
         for (int v : r) { s += v; mx = std::max(mx, v); }
         info->sumRank = s; info->maxBlockRank = mx;
@@ -1267,6 +1668,177 @@ int dmhm_hmat_multiply_dense(dmhm_hmat h, char op, double alpha, const void* X, 
         HIPCHECK(hipStreamSynchronize(g_lib.stream));
     });
 }
+
+int dmhm_rank_model_num_params(void) { return mlp_param_count(kRankFeatures); }
+
+int dmhm_rank_model_create(uint64_t seed, dmhm_rank_model* out)
+{
+    return guard([&] {
+        require_ready();
+        if (!out) throw Error(DMHM_EINVAL, "null pointer");
+        std::unique_ptr<dmhm_rank_model_s> m(new dmhm_rank_model_s());
+        m->nf = kRankFeatures;
+        m->np = mlp_param_count(m->nf);
+        // He initialisation of the weights, zero biases (layout as in k_mlp.hip).
+        std::vector<float> p(m->np, 0.0f);
+        std::mt19937_64 gen(seed);
+        std::normal_distribution<float> nd(0.0f, 1.0f);
+        const int H = 64;
+        float* W1 = p.data();
+        float* W2 = W1 + m->nf * H + H;
+        float* w3 = W2 + H * H + H;
+        for (int e = 0; e < m->nf * H; ++e) W1[e] = nd(gen) * std::sqrt(2.0f / m->nf);
+        for (int e = 0; e < H * H; ++e) W2[e] = nd(gen) * std::sqrt(2.0f / H);
+        for (int e = 0; e < H; ++e) w3[e] = nd(gen) * std::sqrt(1.0f / H);
+        const size_t bytes = (size_t)m->np * sizeof(float);
+        m->params.ensure(bytes); m->grads.ensure(bytes); m->m.ensure(bytes); m->v.ensure(bytes);
+        HIPCHECK(copy_sync(m->params.ptr, p.data(), bytes, hipMemcpyHostToDevice));
+        launch_fill<float>((float*)m->grads.ptr, m->np, 0.0f, g_lib.stream);
+        launch_fill<float>((float*)m->m.ptr, m->np, 0.0f, g_lib.stream);
+        launch_fill<float>((float*)m->v.ptr, m->np, 0.0f, g_lib.stream);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        *out = m.release();
+    });
+}
+
+int dmhm_rank_model_destroy(dmhm_rank_model m)
+{
+    return guard([&] {
+        if (g_lib.ready) HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        delete m;
+    });
+}
+
+int dmhm_rank_model_get_params(dmhm_rank_model m, float* params, int count)
+{
+    return guard([&] {
+        require_ready();
+        if (!m || !params || count != m->np) throw Error(DMHM_EINVAL, "bad parameter buffer");
+        HIPCHECK(copy_sync(params, m->params.ptr, (size_t)count * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int dmhm_rank_model_set_params(dmhm_rank_model m, const float* params, int count)
+{
+    return guard([&] {
+        require_ready();
+        if (!m || !params || count != m->np) throw Error(DMHM_EINVAL, "bad parameter buffer");
+        HIPCHECK(copy_sync(m->params.ptr, params, (size_t)count * sizeof(float), hipMemcpyHostToDevice));
+        m->step = 0;
+        launch_fill<float>((float*)m->m.ptr, m->np, 0.0f, g_lib.stream);
+        launch_fill<float>((float*)m->v.ptr, m->np, 0.0f, g_lib.stream);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
+
+int dmhm_rank_model_gradient(dmhm_rank_model m, const float* X, const float* t, int n, float* grads, float* loss)
+{
+    return guard([&] {
+        require_ready();
+        if (!m || !X || !t || !grads || n <= 0) throw Error(DMHM_EINVAL, "bad arguments");
+        DeviceBuffer dX, dt, dl;
+        dX.ensure((size_t)n * m->nf * sizeof(float)); dt.ensure((size_t)n * sizeof(float)); dl.ensure(64);
+        HIPCHECK(copy_sync(dX.ptr, X, (size_t)n * m->nf * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHECK(copy_sync(dt.ptr, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+        launch_fill<float>((float*)dl.ptr, 1, 0.0f, g_lib.stream);
+        launch_mlp_grad((const float*)m->params.ptr, m->nf, (const float*)dX.ptr, (const float*)dt.ptr, n,
+                        (float*)m->grads.ptr, (float*)dl.ptr, g_lib.stream);
+        HIPCHECK(copy_sync(grads, m->grads.ptr, (size_t)m->np * sizeof(float), hipMemcpyDeviceToHost));
+        if (loss) HIPCHECK(copy_sync(loss, dl.ptr, sizeof(float), hipMemcpyDeviceToHost));
+        launch_fill<float>((float*)m->grads.ptr, m->np, 0.0f, g_lib.stream);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        dX.release(); dt.release(); dl.release();
+    });
+}
+
+int dmhm_rank_model_predict(dmhm_rank_model m, const float* X, int n, float* y)
+{
+    return guard([&] {
+        require_ready();
+        if (!m || !X || !y || n < 0) throw Error(DMHM_EINVAL, "bad arguments");
+        if (n == 0) return;
+        DeviceBuffer dX, dy;
+        dX.ensure((size_t)n * m->nf * sizeof(float)); dy.ensure((size_t)n * sizeof(float));
+        HIPCHECK(copy_sync(dX.ptr, X, (size_t)n * m->nf * sizeof(float), hipMemcpyHostToDevice));
+        launch_mlp_infer((const float*)m->params.ptr, m->nf, (const float*)dX.ptr, n, (float*)dy.ptr, nullptr,
+                         0.0f, 0, 0, 0, g_lib.stream);
+        HIPCHECK(copy_sync(y, dy.ptr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+        dX.release(); dy.release();
+    });
+}
+
+int dmhm_rank_model_train(dmhm_rank_model m, const float* X, const float* t, int n, int steps, int batch, float lr,
+                          uint64_t seed, float* losses)
+{
+    return guard([&] {
+        require_ready();
+        if (!m || !X || !t || n <= 0 || steps < 0 || batch <= 0 || !(lr > 0))
+            throw Error(DMHM_EINVAL, "bad training arguments");
+        // Seeded shuffle of the samples on the host, then everything on the device.
+        std::vector<int> perm(n);
+        for (int i = 0; i < n; ++i) perm[i] = i;
+        std::mt19937_64 gen(seed);
+        std::shuffle(perm.begin(), perm.end(), gen);
+        std::vector<float> hx((size_t)n * m->nf), ht(n);
+        for (int i = 0; i < n; ++i) {
+            std::memcpy(&hx[(size_t)i * m->nf], X + (size_t)perm[i] * m->nf, m->nf * sizeof(float));
+            ht[i] = t[perm[i]];
+        }
+        DeviceBuffer dX, dt, dl;
+        dX.ensure(hx.size() * sizeof(float)); dt.ensure(ht.size() * sizeof(float));
+        dl.ensure((size_t)std::max(steps, 1) * sizeof(float));
+        HIPCHECK(copy_sync(dX.ptr, hx.data(), hx.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHECK(copy_sync(dt.ptr, ht.data(), ht.size() * sizeof(float), hipMemcpyHostToDevice));
+        launch_fill<float>((float*)dl.ptr, std::max(steps, 1), 0.0f, g_lib.stream);
+        const int b = std::min(batch, n);
+        for (int st = 0; st < steps; ++st) {
+            const int off = (int)(((int64_t)st * b) % n);
+            const int nb = std::min(b, n - off);
+            launch_mlp_grad((const float*)m->params.ptr, m->nf, (const float*)dX.ptr + (size_t)off * m->nf,
+                            (const float*)dt.ptr + off, nb, (float*)m->grads.ptr, (float*)dl.ptr + st,
+                            g_lib.stream);
+            launch_mlp_adam((float*)m->params.ptr, (float*)m->grads.ptr, (float*)m->m.ptr, (float*)m->v.ptr, m->np,
+                            lr, ++m->step, g_lib.stream);
+        }
+        if (losses && steps > 0)
+            HIPCHECK(copy_sync(losses, dl.ptr, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        dX.release(); dt.release(); dl.release();
+    });
+}
+
+int dmhm_hmat_rank_features(dmhm_hmat A, float* out, int count)
+{
+    return guard([&] {
+        require_ready();
+        if (!A || !out) throw Error(DMHM_EINVAL, "null pointer");
+        const int nl = (int)A->geo().lrBlocks.size();
+        if (count != nl * kRankFeatures) throw Error(DMHM_EINVAL, "count must be numLowRank x DMHM_RANK_FEATURES");
+        if (nl == 0) return;
+        g_lib.featBuf.ensure((size_t)nl * kRankFeatures * sizeof(float));
+        rank_features(A, static_cast<float*>(g_lib.featBuf.ptr));
+        HIPCHECK(copy_sync(out, g_lib.featBuf.ptr, (size_t)count * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int dmhm_last_product_stats(long long* out)
+{
+    return guard([&] {
+        if (!out) throw Error(DMHM_EINVAL, "null pointer");
+        out[0] = g_lastProduct.blocks; out[1] = g_lastProduct.widthSum;
+        out[2] = g_lastProduct.fallbacks; out[3] = g_lastProduct.fullWidth;
+    });
+}
+
+int dmhm_set_tile_engine(int engine)
+{
+    return guard([&] {
+        if (engine < -1 || engine >= TILE_ENGINES) throw Error(DMHM_EINVAL, "unknown tile engine");
+        g_lib.tileEngine = engine;
+    });
+}
+
+int dmhm_get_tile_engine(void) { return g_lib.tileEngine; }
 
 int dmhm_hmat_multiply_hmat(double alpha, dmhm_hmat A, dmhm_hmat B, dmhm_hmat* C, const dmhm_hh_opts* opts)
 {
@@ -1455,8 +2027,8 @@ static void read_back(dmhm_hmat_s* h, std::vector<char>& vals, std::vector<int>&
     HIPCHECK(hipStreamSynchronize(g_lib.stream));
     vals.resize(L.totalElems() * h->esize());
     ranks.resize(L.nLowRank());
-    if (!vals.empty()) HIPCHECK(hipMemcpy(vals.data(), h->vals, vals.size(), hipMemcpyDeviceToHost));
-    if (!ranks.empty()) HIPCHECK(hipMemcpy(ranks.data(), h->ranks, ranks.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (!vals.empty()) HIPCHECK(copy_sync(vals.data(), h->vals, vals.size(), hipMemcpyDeviceToHost));
+    if (!ranks.empty()) HIPCHECK(copy_sync(ranks.data(), h->ranks, ranks.size() * sizeof(int), hipMemcpyDeviceToHost));
 }
 
 static uint64_t packed_size(dmhm_hmat_s* h, const std::vector<int>& ranks)
@@ -1543,8 +2115,8 @@ static dmhm_hmat_s* unpack_hmat(const char* in, uint64_t size)
         std::memcpy(vals.data() + lay->uOff[i] * es, take(u), u);
         std::memcpy(vals.data() + lay->vOff[i] * es, take(v), v);
     }
-    HIPCHECK(hipMemcpy(h->vals, vals.data(), vals.size(), hipMemcpyHostToDevice));
-    if (!ranks.empty()) HIPCHECK(hipMemcpy(h->ranks, ranks.data(), ranks.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(copy_sync(h->vals, vals.data(), vals.size(), hipMemcpyHostToDevice));
+    if (!ranks.empty()) HIPCHECK(copy_sync(h->ranks, ranks.data(), ranks.size() * sizeof(int), hipMemcpyHostToDevice));
     h->full = hd.full != 0;
     return h.release();
 }
@@ -1583,7 +2155,7 @@ int dmhm_hmat_ranks(dmhm_hmat h, int* ranks, int count)
         if (!h || !ranks) throw Error(DMHM_EINVAL, "null pointer");
         const int n = std::min(count, h->lay->nLowRank());
         HIPCHECK(hipStreamSynchronize(g_lib.stream));
-        if (n > 0) HIPCHECK(hipMemcpy(ranks, h->ranks, n * sizeof(int), hipMemcpyDeviceToHost));
+        if (n > 0) HIPCHECK(copy_sync(ranks, h->ranks, n * sizeof(int), hipMemcpyDeviceToHost));
     });
 }
 
@@ -1639,11 +2211,11 @@ void Plan::upload()
     for (auto& ph : phases) {
         if (!ph.tiles.empty()) {
             HIPCHECK(hipMalloc(&ph.dTiles, ph.tiles.size() * sizeof(Tile)));
-            HIPCHECK(hipMemcpy(ph.dTiles, ph.tiles.data(), ph.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+            HIPCHECK(copy_sync(ph.dTiles, ph.tiles.data(), ph.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
         }
         if (!ph.contribs.empty()) {
             HIPCHECK(hipMalloc(&ph.dContribs, ph.contribs.size() * sizeof(Contrib)));
-            HIPCHECK(hipMemcpy(ph.dContribs, ph.contribs.data(), ph.contribs.size() * sizeof(Contrib),
+            HIPCHECK(copy_sync(ph.dContribs, ph.contribs.data(), ph.contribs.size() * sizeof(Contrib),
                                hipMemcpyHostToDevice));
         }
         const void* src = nullptr; size_t bytes = 0;
@@ -1686,13 +2258,13 @@ void Plan::upload()
             src = ph.mvAtoms.data(); bytes = ph.mvAtoms.size() * sizeof(MvAtom);
             HIPCHECK(hipMalloc(&ph.dAux, std::max<size_t>(ph.mvContribs.size(), 1) * sizeof(MvContrib)));
             if (!ph.mvContribs.empty())
-                HIPCHECK(hipMemcpy(ph.dAux, ph.mvContribs.data(), ph.mvContribs.size() * sizeof(MvContrib),
+                HIPCHECK(copy_sync(ph.dAux, ph.mvContribs.data(), ph.mvContribs.size() * sizeof(MvContrib),
                                    hipMemcpyHostToDevice));
             std::vector<MvContrib>().swap(ph.mvContribs);
         }
         if (src) {
             HIPCHECK(hipMalloc(&ph.dTasks, bytes));
-            HIPCHECK(hipMemcpy(ph.dTasks, src, bytes, hipMemcpyHostToDevice));
+            HIPCHECK(copy_sync(ph.dTasks, src, bytes, hipMemcpyHostToDevice));
         }
         // Host copies of the big arrays are no longer needed.
         std::vector<Contrib>().swap(ph.contribs);

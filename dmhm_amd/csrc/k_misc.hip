@@ -23,10 +23,20 @@ __device__ inline double gauss_at(uint64_t seed, uint64_t stream, uint64_t idx)
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+// Seeds may carry a device-resident offset (`seedOff`, or null): an iteration
+// counter that a captured HIP graph advances on every replay, so replays draw
+// fresh sketches exactly as the eager loop does. The counter is spread by an
+// odd 64-bit multiplier so that seeds of different roles (base + 0, 1, 2)
+// never meet across iterations.
+__device__ inline uint64_t seed_with(uint64_t seed0, const uint64_t* seedOff)
+{
+    return seed0 + (seedOff ? *seedOff * 0x9E3779B97F4A7C15ULL : 0);
+}
 template <typename T>
 __global__ void __launch_bounds__(256) rng_kernel(Bases b, const RngTask* __restrict__ tasks,
-                                                  uint64_t seed)
+                                                  uint64_t seed0, const uint64_t* seedOff)
 {
+    const uint64_t seed = seed_with(seed0, seedOff);
     const RngTask tk = tasks[blockIdx.x];
     T* x = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.off;
     const int64_t n = (int64_t)tk.m * tk.k;
@@ -34,25 +44,27 @@ __global__ void __launch_bounds__(256) rng_kernel(Bases b, const RngTask* __rest
 }
 
 template <typename T>
-void launch_rng_phase(const Phase& ph, const Bases& b, uint64_t seed, hipStream_t s)
+void launch_rng_phase(const Phase& ph, const Bases& b, uint64_t seed, hipStream_t s, const uint64_t* seedOff)
 {
     const int n = (int)ph.rng.size();
     if (n == 0) return;
-    rng_kernel<T><<<n, 256, 0, s>>>(b, static_cast<const RngTask*>(ph.dTasks), seed);
+    rng_kernel<T><<<n, 256, 0, s>>>(b, static_cast<const RngTask*>(ph.dTasks), seed, seedOff);
     HIP_LAUNCH_CHECK();
 }
 
 template <typename T>
-__global__ void gaussian_kernel(T* x, int64_t n, uint64_t seed, uint64_t stream)
+__global__ void gaussian_kernel(T* x, int64_t n, uint64_t seed0, uint64_t stream, const uint64_t* seedOff)
 {
+    const uint64_t seed = seed_with(seed0, seedOff);
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
         x[e] = (T)gauss_at(seed, stream, (uint64_t)e);
 }
 
-template <typename T> void launch_gaussian(T* x, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s)
+template <typename T> void launch_gaussian(T* x, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s,
+                                           const uint64_t* seedOff)
 {
     if (n <= 0) return;
-    gaussian_kernel<T><<<grid_for(n), 256, 0, s>>>(x, n, seed, stream);
+    gaussian_kernel<T><<<grid_for(n), 256, 0, s>>>(x, n, seed, stream, seedOff);
     HIP_LAUNCH_CHECK();
 }
 
@@ -254,6 +266,13 @@ template <typename T> void launch_normalize(T* x, int64_t n, const double* sumsq
     HIP_LAUNCH_CHECK();
 }
 
+__global__ void bump_kernel(uint64_t* c) { if (threadIdx.x == 0) *c += 1; }
+void launch_bump(uint64_t* counter, hipStream_t s)
+{
+    bump_kernel<<<1, 64, 0, s>>>(counter);
+    HIP_LAUNCH_CHECK();
+}
+
 __global__ void zero_double_kernel(double* x, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -267,12 +286,12 @@ void launch_zero_double(double* x, int n, hipStream_t s)
 
 // Explicit instantiations.
 #define INST(T)                                                                                       \
-    template void launch_rng_phase<T>(const Phase&, const Bases&, uint64_t, hipStream_t);           \
+    template void launch_rng_phase<T>(const Phase&, const Bases&, uint64_t, hipStream_t, const uint64_t*); \
     template void launch_scale<T>(T*, int64_t, T, hipStream_t);                                     \
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                      \
     template void launch_eye<T>(T*, int, hipStream_t);                                              \
     template void launch_axpby<T>(const T*, const T*, T, T*, int64_t, hipStream_t);                 \
-    template void launch_gaussian<T>(T*, int64_t, uint64_t, uint64_t, hipStream_t);                 \
+    template void launch_gaussian<T>(T*, int64_t, uint64_t, uint64_t, hipStream_t, const uint64_t*); \
     template void launch_add_diag<T>(T*, const int64_t*, const int*, int, T, hipStream_t);          \
     template void launch_transpose_dense<T>(const T*, T*, const int64_t*, const int64_t*, const int*, \
                                             const int*, int, hipStream_t);                          \

@@ -45,9 +45,10 @@ __global__ void __launch_bounds__(256) qr_kernel(Bases b, const QrTask* __restri
     __shared__ double scale_s;
     const QrTask tk = tasks[blockIdx.x];
     T* A = reinterpret_cast<T*>(b.val[tk.slot]) + tk.off;
-    const int m = tk.m, k = tk.k, kk = min(m, k);
+    const int m = tk.m, kstat = tk.k;
+    const int k = tk.krank >= 0 ? min(kstat, b.rank[tk.kslot][tk.krank]) : kstat, kk = min(m, k);
     const int tid = threadIdx.x;
-    if (m <= 0) return;
+    if (m <= 0 || k <= 0) return;
     // Factor.
     for (int j = 0; j < kk; ++j) {
         column_dots(A, m, j, j, k, dots, red);
@@ -80,12 +81,12 @@ __global__ void __launch_bounds__(256) qr_kernel(Bases b, const QrTask* __restri
         }
         __syncthreads();
     }
-    // R (k x k, upper triangle; rows >= kk zero).
+    // R (kstat x k, ld rld: upper triangle, rows >= kk zero).
     if (tk.roff >= 0) {
         T* R = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.roff;
-        for (int e = tid; e < k * k; e += 256) {
-            const int r = e % k, c = e / k;
-            R[e] = (r <= c && r < kk) ? A[r + (int64_t)c * m] : T(0);
+        for (int e = tid; e < kstat * k; e += 256) {
+            const int r = e % kstat, c = e / kstat;
+            R[r + (int64_t)c * tk.rld] = (r <= c && r < kk) ? A[r + (int64_t)c * m] : T(0);
         }
     }
     __syncthreads();
@@ -145,7 +146,10 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
     __shared__ double taus[KMAX];
     const QrTask tk = tasks[blockIdx.x];
     T* A = reinterpret_cast<T*>(b.val[tk.slot]) + tk.off;
-    const int m = tk.m, k = tk.k, kk = min(m, k), ld = tk.ld;
+    const int m = tk.m, kstat = tk.k, ld = tk.ld;
+    // Effective width (width-indirect sketches): columns >= k are left alone.
+    const int k = tk.krank >= 0 ? min(kstat, b.rank[tk.kslot][tk.krank]) : kstat, kk = min(m, k);
+    if (k <= 0) return;
     const int i = threadIdx.x, w = i >> 6, lane = i & 63;
     const bool inPanel = i < m;
     double a[KMAX];
@@ -200,7 +204,9 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
         if (i == 0) taus[j] = t;
     }
     __syncthreads();
-    if (tk.roff >= 0 && i < k) {
+    // R: rows 0..kstat-1 of the first k columns (rows >= kk zero, so stacked R
+    // factors of a TSQR carry zero rows below a narrower effective width).
+    if (tk.roff >= 0 && i < kstat) {
         T* R = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.roff;
 #pragma unroll
         for (int c = 0; c < KMAX; ++c)

@@ -21,11 +21,13 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
     __shared__ int rotated;
     __shared__ int keep_s;
     const SvdTask tk = tasks[blockIdx.x];
-    const int k = tk.k, tid = threadIdx.x;
+    const int ld = tk.ld > 0 ? tk.ld : tk.k;
+    const int k = tk.krank >= 0 ? min(tk.k, b.rank[SLOT_WIDTH][tk.krank]) : tk.k;
+    const int tid = threadIdx.x;
     T* ws = reinterpret_cast<T*>(b.val[SLOT_WS]);
     for (int e = tid; e < k * k; e += NT) {
         const int r = e % k, c = e / k;
-        As[c][r] = (double)ws[tk.roff + e];
+        As[c][r] = (double)ws[tk.roff + r + (int64_t)c * ld];
         Vs[c][r] = (r == c) ? 1.0 : 0.0;
     }
     const int kp = k + (k & 1);
@@ -84,7 +86,7 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
                 if (sig[order[c]] > sig[order[best]]) best = c;
             const int tmp = order[a]; order[a] = order[best]; order[best] = tmp;
         }
-        int keep = tk.maxKeep;
+        int keep = min(tk.maxKeep, k);
         if (tolMode == 1) {
             const double s0 = sig[order[0]];
             if (s0 < relTol * relTol) keep = min(1, keep);
@@ -96,19 +98,25 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
         }
         keep_s = keep;
         b.rank[SLOT_C][tk.rankIndex] = keep;
+        // A-posteriori check of a narrowed sketch: fewer than `spare` of its
+        // singular values fell below the truncation rule, so the block may
+        // hold more numerical rank than the sketch saw.
+        if (tk.krank >= 0 && k < tk.k && keep > k - tk.spare && b.rank[SLOT_FLAG])
+            b.rank[SLOT_FLAG][tk.rankIndex] = 1;
     }
     __syncthreads();
     const int keep = keep_s;
-    for (int e = tid; e < k * tk.cap; e += NT) {
-        const int r = e % k, j = e / k;
+    const int kst = tk.k;                 // rows k..kst-1 of G1, G2 are zero
+    for (int e = tid; e < kst * tk.cap; e += NT) {
+        const int r = e % kst, j = e / kst;
         double g1 = 0.0, g2 = 0.0;
-        if (j < keep) {
+        if (j < keep && r < k) {
             const int c = order[j];
             g1 = Vs[c][r] * sig[c];
             g2 = sig[c] > 0 ? As[c][r] / sig[c] : 0.0;
         }
-        ws[tk.g1off + e] = (T)g1;
-        ws[tk.g2off + e] = (T)g2;
+        ws[tk.g1off + r + (int64_t)j * ld] = (T)g1;
+        ws[tk.g2off + r + (int64_t)j * ld] = (T)g2;
     }
 }
 

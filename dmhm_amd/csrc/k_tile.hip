@@ -175,6 +175,221 @@ __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// MFMA engine: one wave per 16-row block of an output tile. A phase of 64-row
+// tiles runs one workgroup of four waves per tile (wave w owns rows
+// 16w..16w+15), a phase of tiles of at most 16 rows packs four tiles per
+// workgroup; no LDS and no barriers, so every wave streams its contributions
+// independently. A wave computes its block transposed, D^T = R^T L^T, in
+// 16 x 16 MFMA blocks: the first operand is a 16-column block of R^T, the
+// second the wave's 16-row block of L^T, and lane l holds
+// D(row0 + l%16, cb*16 + c(l, r)) in accumulator register r, so each group of
+// 16 lanes stores 16 consecutive rows of a column. The FP32 and BF16
+// instructions place output row 4*(l/16) + r of the 16 x 16 block in register
+// r of lane l, the FP64 one row l/16 + 4*r (measured on gfx950), hence
+// c(l, r) = 4*(l/16) + r for FP32 accumulators and l/16 + 4*r for FP64. The
+// inner dimension advances in chunks of 16; lane l supplies inner indices
+// p0 + 4*(l/16) + e, e < 4, to both operands: four K = 4 steps of
+// v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 (MODE 0, arithmetic in T), or
+// one K = 16 step of v_mfma_f32_16x16x16_bf16 on operands rounded from FP32 to
+// BF16 with FP32 accumulation (MODE 1). Out-of-range entries load as zero.
+// PF = 1 issues the loads of the next chunk (possibly of the next
+// contribution) before the MFMAs of the current one.
+// ---------------------------------------------------------------------------
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mf_b4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MfAcc;
+template <> struct MfAcc<double> { typedef mf_d4 type; };
+template <> struct MfAcc<float> { typedef mf_f4 type; };
+
+__device__ __forceinline__ mf_d4 mf_step(double a, double b, mf_d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mf_f4 mf_step(float a, float b, mf_f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mf_f4 mf_step_bf16(const float (&a)[4], const float (&b)[4], mf_f4 c)
+{
+    mf_b4 x, y;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) { x[e] = (__bf16)a[e]; y[e] = (__bf16)b[e]; }
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, y, c, 0, 0, 0);
+}
+
+// Operands of one contribution as the wave reads them.
+template <typename T>
+struct MfSrc {
+    const T* L;
+    const T* R;
+    int64_t ldl, ldr;
+    int q;
+    T a;
+    bool tL, tR;
+};
+
+template <typename T>
+__device__ __forceinline__ MfSrc<T> mf_src(const Bases& b, const Contrib& C, int q, T alpha, T beta)
+{
+    MfSrc<T> s;
+    s.L = reinterpret_cast<const T*>(b.val[C.lslot]) + C.loff;
+    s.R = reinterpret_cast<const T*>(b.val[C.rslot]) + C.roff;
+    s.ldl = C.ldl; s.ldr = C.ldr; s.q = q;
+    s.a = (C.flags & CF_RUNTIME_ALPHA) ? alpha * T(C.alpha)
+        : (C.flags & CF_RUNTIME_BETA) ? beta * T(C.alpha) : T(C.alpha);
+    s.tL = C.flags & CF_TRANS_L; s.tR = C.flags & CF_TRANS_R;
+    return s;
+}
+
+// Fragments of chunk p0: lf[e] = L(row, p), rf[cb][e] = a R(p, cb*16 + l%16),
+// p = p0 + 4*(l/16) + e.
+template <typename T, int NBMAX>
+__device__ __forceinline__ void mf_load(const MfSrc<T>& s, int p0, int row, int m, int k, int NB, int li, int lk,
+                                        T (&lf)[4], T (&rf)[NBMAX][4])
+{
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int p = p0 + 4 * lk + e;
+        const bool pin = p < s.q;
+        lf[e] = (pin && row < m) ? (s.tL ? s.L[p + row * s.ldl] : s.L[row + p * s.ldl]) : T(0);
+#pragma unroll
+        for (int cb = 0; cb < NBMAX; ++cb) {
+            const int col = cb * 16 + li;
+            rf[cb][e] = (cb < NB && pin && col < k) ? s.a * (s.tR ? s.R[col + p * s.ldr] : s.R[p + col * s.ldr])
+                                                    : T(0);
+        }
+    }
+}
+
+template <typename T, int MODE, int NBMAX, typename Acc>
+__device__ __forceinline__ void mf_mma(const T (&lf)[4], const T (&rf)[NBMAX][4], int NB, Acc (&acc)[NBMAX])
+{
+#pragma unroll
+    for (int cb = 0; cb < NBMAX; ++cb) {
+        if (cb >= NB) continue;
+        if constexpr (MODE == 1) {
+            acc[cb] = mf_step_bf16(rf[cb], lf, acc[cb]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[cb] = mf_step(rf[cb][e], lf[e], acc[cb]);
+        }
+    }
+}
+
+// First contribution at or after c with a non-empty inner dimension.
+__device__ __forceinline__ int mf_next(const Bases& b, const Contrib* __restrict__ cs, int c, int ce, int& q)
+{
+    for (; c < ce; ++c) {
+        q = contrib_q(b, cs[c]);
+        if (q > 0) break;
+    }
+    return c;
+}
+
+template <typename T, int MODE, int MBMAX, int NBMAX, int PF>
+__global__ void __launch_bounds__(256) tile_mfma_kernel(Bases b, const Tile* __restrict__ tiles,
+                                                        const Contrib* __restrict__ cs, int ntiles, T alpha,
+                                                        T beta)
+{
+    static_assert(MODE == 0 || sizeof(T) == 4, "the BF16 mode reads FP32 operands");
+    static_assert(MBMAX == 1 || MBMAX == 4, "tiles of at most 16 or 64 rows");
+    typedef typename MfAcc<T>::type Acc;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int id = MBMAX == 4 ? (int)blockIdx.x : (int)blockIdx.x * 4 + w;
+    const int rb = MBMAX == 4 ? w : 0;
+    if (id >= ntiles) return;
+    const Tile tl = tiles[id];
+    int m = tl.m, k = tl.k;
+    if (tl.mrank >= 0) m = min(m, max(0, b.rank[tl.mslot][tl.mrank] - tl.moff));
+    if (tl.krank >= 0) k = min(k, max(0, b.rank[tl.kslot][tl.krank] - tl.koff));
+    if (m <= rb * 16 || k <= 0) return;
+    const int NB = (k + 15) >> 4;
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int row = rb * 16 + li;
+    Acc acc[NBMAX];
+#pragma unroll
+    for (int cb = 0; cb < NBMAX; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[cb][r] = T(0);
+
+    int q = 0;
+    int c = mf_next(b, cs, tl.cb, tl.ce, q);
+    if (c < tl.ce) {
+        MfSrc<T> src = mf_src<T>(b, cs[c], q, alpha, beta);
+        int p0 = 0;
+        T lf[4], rf[NBMAX][4];
+        mf_load<T, NBMAX>(src, p0, row, m, k, NB, li, lk, lf, rf);
+        while (true) {
+            // The next chunk: p0 + 16 of this contribution, or the next one.
+            int p1 = p0 + 16, c1 = c, q1 = src.q;
+            if (p1 >= src.q) { p1 = 0; c1 = mf_next(b, cs, c + 1, tl.ce, q1); }
+            const bool more = c1 < tl.ce;
+            MfSrc<T> src1 = src;
+            if (more && c1 != c) src1 = mf_src<T>(b, cs[c1], q1, alpha, beta);
+            if constexpr (PF == 1) {
+                T lf1[4], rf1[NBMAX][4];
+                if (more) mf_load<T, NBMAX>(src1, p1, row, m, k, NB, li, lk, lf1, rf1);
+                mf_mma<T, MODE, NBMAX>(lf, rf, NB, acc);
+                if (!more) break;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    lf[e] = lf1[e];
+#pragma unroll
+                    for (int cb = 0; cb < NBMAX; ++cb) rf[cb][e] = rf1[cb][e];
+                }
+            } else {
+                mf_mma<T, MODE, NBMAX>(lf, rf, NB, acc);
+                if (!more) break;
+                mf_load<T, NBMAX>(src1, p1, row, m, k, NB, li, lk, lf, rf);
+            }
+            c = c1; p0 = p1; src = src1;
+        }
+    }
+    if (row >= m) return;
+    T* out = reinterpret_cast<T*>(b.val[tl.slot]) + tl.off;
+    const bool useBeta = (tl.flags & TF_RUNTIME_BETA) && beta != T(0);
+    const bool accum = (tl.flags & TF_ACCUMULATE);
+    const int64_t ld = tl.ld;
+#pragma unroll
+    for (int cb = 0; cb < NBMAX; ++cb) {
+        if (cb >= NB) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int col = cb * 16 + (sizeof(T) == 8 ? lk + 4 * r : 4 * lk + r);
+            if (col < k) {
+                T* o = out + row + col * ld;
+                const T v = acc[cb][r];
+                if (useBeta) *o = beta * *o + v;
+                else if (accum) *o += v;
+                else *o = v;
+            }
+        }
+    }
+}
+
+template <typename T, int MODE, int MBMAX, int PF>
+static void launch_mfma_mb(int n, int kmax, const Bases& b, const Tile* t, const Contrib* c, T alpha, T beta,
+                           hipStream_t s)
+{
+    const int grid = MBMAX == 4 ? n : (n + 3) / 4;
+    if (kmax <= 16) tile_mfma_kernel<T, MODE, MBMAX, 1, PF><<<grid, 256, 0, s>>>(b, t, c, n, alpha, beta);
+    else if (kmax <= 32) tile_mfma_kernel<T, MODE, MBMAX, 2, PF><<<grid, 256, 0, s>>>(b, t, c, n, alpha, beta);
+    else tile_mfma_kernel<T, MODE, MBMAX, 4, PF><<<grid, 256, 0, s>>>(b, t, c, n, alpha, beta);
+}
+
+template <typename T, int MODE, int PF>
+static void launch_mfma(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s)
+{
+    const int n = (int)ph.tiles.size();
+    const Tile* t = static_cast<const Tile*>(ph.dTiles);
+    const Contrib* c = static_cast<const Contrib*>(ph.dContribs);
+    if (ph.maxMTile <= 16) launch_mfma_mb<T, MODE, 1, PF>(n, ph.maxKTile, b, t, c, alpha, beta, s);
+    else launch_mfma_mb<T, MODE, 4, PF>(n, ph.maxKTile, b, t, c, alpha, beta, s);
+}
+
 template <typename T, int ROLE, int MR>
 static void launch_tile_mr(int n, int kmax, const Bases& b, const Tile* t, const Contrib* c, T alpha, T beta,
                            hipStream_t s)
@@ -186,7 +401,7 @@ static void launch_tile_mr(int n, int kmax, const Bases& b, const Tile* t, const
 }
 
 template <typename T>
-void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s)
+void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s, int engine)
 {
     const int n = (int)ph.tiles.size();
     if (n == 0) return;
@@ -194,6 +409,23 @@ void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStre
     const Contrib* c = static_cast<const Contrib*>(ph.dContribs);
     const int kmax = ph.maxKTile;
     if (kmax > 64 || ph.maxMTile > 64) throw std::runtime_error("tile larger than 64 x 64");
+    if (engine != TILE_LDS) {
+        const bool bf16 = (engine == TILE_MFMA_BF16 || engine == TILE_MFMA_BF16_PF) && sizeof(T) == 4;
+        // The prefetching form runs in FP32 only: in FP64 it produced wrong
+        // products on gfx950 (the non-prefetching form is exact).
+        const bool pf = (engine == TILE_MFMA_PF || engine == TILE_MFMA_BF16_PF) && sizeof(T) == 4;
+        if (bf16) {
+            if constexpr (sizeof(T) == 4) {
+                if (pf) launch_mfma<float, 1, 1>(ph, b, alpha, beta, s);
+                else launch_mfma<float, 1, 0>(ph, b, alpha, beta, s);
+            }
+        } else {
+            if (pf) launch_mfma<T, 0, 1>(ph, b, alpha, beta, s);
+            else launch_mfma<T, 0, 0>(ph, b, alpha, beta, s);
+        }
+        HIP_LAUNCH_CHECK();
+        return;
+    }
     // ROLE only distinguishes projection and accumulation launches in profiles;
     // MR = 16 serves phases whose tiles all have at most 16 rows.
     if (ph.maxMTile <= 16) {
@@ -206,7 +438,7 @@ void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStre
     HIP_LAUNCH_CHECK();
 }
 
-template void launch_tile_phase<double>(const Phase&, const Bases&, double, double, hipStream_t);
-template void launch_tile_phase<float>(const Phase&, const Bases&, float, float, hipStream_t);
+template void launch_tile_phase<double>(const Phase&, const Bases&, double, double, hipStream_t, int);
+template void launch_tile_phase<float>(const Phase&, const Bases&, float, float, hipStream_t, int);
 
 } // namespace dmhm

@@ -17,8 +17,15 @@ struct TruncPolicy {
     double relTol;
 };
 
+// Tile engines: TILE_LDS, the workgroup-per-tile kernel staging operands
+// through LDS (vector FMA); TILE_MFMA, the wave-per-row-block MFMA kernel in
+// the storage precision; TILE_MFMA_BF16, the same with FP32 operands rounded
+// to BF16 and FP32 accumulation (FP64 storage falls back to TILE_MFMA); the
+// _PF forms prefetch the next chunk before the current one's MFMAs.
+enum TileEngine { TILE_LDS = 0, TILE_MFMA = 1, TILE_MFMA_PF = 2, TILE_MFMA_BF16 = 3, TILE_MFMA_BF16_PF = 4,
+                  TILE_ENGINES = 5 };
 template <typename T>
-void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s);
+void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s, int engine);
 template <typename T>
 void launch_qr_phase(const Phase& ph, const Bases& b, hipStream_t s);
 template <typename T>
@@ -26,7 +33,8 @@ void launch_svd_phase(const Phase& ph, const Bases& b, TruncPolicy pol, hipStrea
 template <typename T>
 void launch_mv_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s);
 template <typename T>
-void launch_rng_phase(const Phase& ph, const Bases& b, uint64_t seed, hipStream_t s);
+void launch_rng_phase(const Phase& ph, const Bases& b, uint64_t seed, hipStream_t s,
+                      const uint64_t* seedOff = nullptr);
 
 // Elementwise helpers over raw device arrays.
 template <typename T> void launch_scale(T* x, int64_t n, T alpha, hipStream_t s);
@@ -37,7 +45,11 @@ void launch_fill_int(int* x, int64_t n, int v, hipStream_t s);
 // x (n x n, ld n) = identity
 template <typename T> void launch_eye(T* x, int n, hipStream_t s);
 template <typename Ts, typename Td> void launch_convert(const Ts* x, Td* y, int64_t n, hipStream_t s);
-template <typename T> void launch_gaussian(T* x, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s);
+// Seeds add the device word *seedOff when it is given (graph-replayed loops).
+template <typename T> void launch_gaussian(T* x, int64_t n, uint64_t seed, uint64_t stream, hipStream_t s,
+                                           const uint64_t* seedOff = nullptr);
+// *counter += 1 on the device (the iteration counter of a replayed loop).
+void launch_bump(uint64_t* counter, hipStream_t s);
 
 // Per-leaf operations driven by small index tables on the device.
 // add alpha to the diagonal of square dense leaves: offsets + sizes.
@@ -60,5 +72,17 @@ template <typename T> void launch_sumsq(const T* x, int64_t n, double* out, hipS
 template <typename T> void launch_normalize(T* x, int64_t n, const double* sumsq, double* logAcc,
                                             double root, hipStream_t s);
 void launch_zero_double(double* x, int n, hipStream_t s);
+
+// Rank model (k_mlp.hip): 3-layer perceptron, NF inputs, 64 ReLU units per
+// hidden layer; mlp_param_count(NF) floats of parameters.
+int mlp_param_count(int nf);
+void launch_mlp_infer(const float* params, int nf, const float* X, int n, float* y, int* widths, float margin,
+                      int extra, int wmin, int wmax, hipStream_t s);
+void launch_mlp_grad(const float* params, int nf, const float* X, const float* t, int n, float* grads,
+                     float* loss, hipStream_t s);
+void launch_mlp_adam(float* p, float* g, float* m, float* v, int np, float lr, int step, hipStream_t s);
+void launch_rank_features(const float* stat, int nstat, const int* prev, float invMax, float* out, int n,
+                          hipStream_t s);
+void launch_widen_flagged(int* widths, int* flags, int n, int wmax, int* count, long long* wsum, hipStream_t s);
 
 } // namespace dmhm

@@ -558,11 +558,11 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     };
     // Projection tile P = F^T X over the atoms of box `bx` (F rows = bx rows).
     auto projTile = [&](PhaseBuilder& pb, const Operand& F, const Box& bx, const Operand& X,
-                        const WFactor& w, int64_t P) {
+                        const WFactor& w, int64_t P, int wr) {
         Tile t{};
         t.off = P; t.ld = w.cap;
         t.m = std::min(64, w.cap); t.mrank = w.rankIndex; t.moff = 0; t.mslot = w.rankSlot;
-        t.k = kS; t.krank = -1; t.slot = SLOT_WS; t.flags = 0;
+        t.k = kS; t.krank = wr; t.koff = 0; t.kslot = SLOT_WIDTH; t.slot = SLOT_WS; t.flags = 0;
         if (w.cap > 64) throw std::invalid_argument("low-rank capacity above 64 in a product");
         const int tile = pb.addTile(t);
         for (int a = bx.atomBegin; a < bx.atomEnd; ++a) {
@@ -612,13 +612,15 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     // workgroup are factored as a TSQR: QR of row blocks of whole atoms
     // (their R factors stacked), QR of the stack, then Q_i := Q_i Q_stack,i.
     const int tsqrRows = tsqrRows_ > 0 ? tsqrRows_ : (kS <= 24 ? 1024 : 512);
-    auto addQr = [&](Phase& q1, Phase& q2, PhaseBuilder& comb, int64_t off, int level, int box, int64_t roff) {
+    auto addQr = [&](Phase& q1, Phase& q2, PhaseBuilder& comb, int64_t off, int level, int box, int64_t roff,
+                     int wr) {
         const Box& B = g.boxes[level][box];
         const int m = B.size;
         plan->flops += 4.0 * m * kS * kS;
         if (m <= tsqrRows || kS > 40) {
             QrTask t{};
             t.off = off; t.roff = roff; t.m = m; t.k = kS; t.ld = m; t.rld = kS; t.slot = SLOT_WS;
+            t.krank = wr; t.kslot = SLOT_WIDTH;
             q1.qr.push_back(t);
             return;
         }
@@ -636,26 +638,29 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             QrTask t{};
             t.off = off + r0; t.roff = Rs + (int64_t)i * kS; t.m = r1 - r0; t.k = kS;
             t.ld = m; t.rld = nb * kS; t.slot = SLOT_WS;
+            t.krank = wr; t.kslot = SLOT_WIDTH;
             q1.qr.push_back(t);
         }
         QrTask t{};
         t.off = Rs; t.roff = roff; t.m = nb * kS; t.k = kS; t.ld = nb * kS; t.rld = kS; t.slot = SLOT_WS;
+        t.krank = wr; t.kslot = SLOT_WIDTH;
         q2.qr.push_back(t);
         for (int i = 0; i < nb; ++i)
             for (int a = blockAtom[i]; a < blockAtom[i + 1]; ++a) {
                 const int ra = g.atomStart[a] - B.off;
                 Tile tl{};
                 tl.off = off + ra; tl.ld = m; tl.m = g.atomSize[a]; tl.k = kS;
-                tl.mrank = tl.krank = -1; tl.slot = SLOT_WS;
+                tl.mrank = -1; tl.krank = wr; tl.koff = 0; tl.kslot = SLOT_WIDTH; tl.slot = SLOT_WS;
                 const int id = comb.addTile(tl);
-                comb.add(id, mkc(SLOT_WS, off + ra, m, SLOT_WS, Rs + (int64_t)i * kS, nb * kS, kS, -1, 0, 0, 1.0f));
+                comb.add(id, mkc(SLOT_WS, off + ra, m, SLOT_WS, Rs + (int64_t)i * kS, nb * kS, kS, wr, SLOT_WIDTH,
+                                 0, 1.0f));
                 plan->flops += 2.0 * g.atomSize[a] * kS * kS;
             }
     };
 
     // 4. Per batch: W factors, dense accumulation, Y = T Omega, QR, Z = T^T Q,
     //    QR + SVD, formation of C.U and C.V.
-    struct Sk { int b; int64_t omega, Y, Z, R, G1, G2, D; int m, n; bool leaf; };
+    struct Sk { int b; int64_t omega, Y, Z, R, G1, G2, D; int m, n; bool leaf; int wr; };
     for (const auto& inGroup : groups) {
         ws_ = globalEnd;
         std::vector<Sk> sk;
@@ -666,6 +671,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             const Box& sb = g.boxes[C.level][C.s];
             Sk s;
             s.b = lbk; s.m = tb.size; s.n = sb.size;
+            s.wr = widthIndirect_ ? C.lrIndex : -1;      // per-block sketch width (rank array index)
             // Leaf-level admissible blocks are accumulated densely, then compressed.
             s.leaf = (C.level == g.numLevels - 1);
             s.D = s.leaf ? wsAlloc((int64_t)tb.size * sb.size) : -1;
@@ -774,7 +780,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 const Block& C = g.blocks[s.b];
                 const Box& tb = g.boxes[C.level][C.t];
                 const Box& sb = g.boxes[C.level][C.s];
-                PanelTiles pt = addPanelTiles(acc, {SLOT_WS, s.Y, s.m}, C.level, C.t, kS, -1, 0, 0);
+                PanelTiles pt = addPanelTiles(acc, {SLOT_WS, s.Y, s.m}, C.level, C.t, kS, s.wr, SLOT_WIDTH, 0);
                 const Operand om{SLOT_WS, s.omega, s.n};
                 if (s.leaf) {
                     for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
@@ -788,7 +794,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 for (int wi : leafW[s.b]) {
                     const WFactor& w = W[wi];
                     const int64_t P = wsAlloc((int64_t)w.cap * kS);
-                    projTile(proj, slicePV(w, sb), sb, om, w, P);
+                    projTile(proj, slicePV(w, sb), sb, om, w, P, s.wr);
                     const Operand pu = slicePU(w, tb);
                     for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
                         const int ra = g.atomStart[a] - tb.off;
@@ -801,8 +807,8 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 for (const auto& tm : nnTerms[s.b]) {
                     const Block& B = g.blocks[tm.second];
                     const Operand in{SLOT_WS, t1Of[(uint64_t)tm.second], g.boxes[B.level][B.t].size};
-                    subh(proj, acc, SLOT_A, LA, tm.first, false, in, B.level, B.t, pt, kS, -1, 0, 1.0f, abFlag,
-                         &plan->flops);
+                    subh(proj, acc, SLOT_A, LA, tm.first, false, in, B.level, B.t, pt, kS, s.wr, SLOT_WIDTH, 1.0f,
+                         abFlag, &plan->flops);
                 }
             }
             proj.finalize();
@@ -817,7 +823,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             PhaseBuilder comb(cb);
             for (const Sk& s : sk) {
                 const Block& C = g.blocks[s.b];
-                addQr(q, q2, comb, s.Y, C.level, C.t, -1);
+                addQr(q, q2, comb, s.Y, C.level, C.t, -1, s.wr);
             }
             comb.finalize();
         }
@@ -834,7 +840,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 const Block& C = g.blocks[s.b];
                 const Box& tb = g.boxes[C.level][C.t];
                 const Box& sb = g.boxes[C.level][C.s];
-                PanelTiles zt = addPanelTiles(acc4, {SLOT_WS, s.Z, s.n}, C.level, C.s, kS, -1, 0, 0);
+                PanelTiles zt = addPanelTiles(acc4, {SLOT_WS, s.Z, s.n}, C.level, C.s, kS, s.wr, SLOT_WIDTH, 0);
                 const Operand Q{SLOT_WS, s.Y, s.m};
                 if (s.leaf) {
                     // Z = D^T Q: L(i, p) = D(p, i)
@@ -850,7 +856,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 for (int wi : leafW[s.b]) {
                     const WFactor& w = W[wi];
                     const int64_t P = wsAlloc((int64_t)w.cap * kS);
-                    projTile(proj3, slicePU(w, tb), tb, Q, w, P);
+                    projTile(proj3, slicePU(w, tb), tb, Q, w, P, s.wr);
                     const Operand pv = slicePV(w, sb);
                     for (int a = sb.atomBegin; a < sb.atomEnd; ++a) {
                         const int ra = g.atomStart[a] - sb.off;
@@ -864,11 +870,12 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                     const Block& A = g.blocks[tm.first];
                     const Box& ub = g.boxes[A.level][A.s];
                     const int64_t T2 = wsAlloc((int64_t)ub.size * kS);
-                    PanelTiles pt = addPanelTiles(acc3, {SLOT_WS, T2, ub.size}, A.level, A.s, kS, -1, 0, 0);
-                    subh(proj3, acc3, SLOT_A, LA, tm.first, true, Q, C.level, C.t, pt, kS, -1, 0, 1.0f, 0,
+                    PanelTiles pt = addPanelTiles(acc3, {SLOT_WS, T2, ub.size}, A.level, A.s, kS, s.wr,
+                                                  SLOT_WIDTH, 0);
+                    subh(proj3, acc3, SLOT_A, LA, tm.first, true, Q, C.level, C.t, pt, kS, s.wr, SLOT_WIDTH, 1.0f, 0,
                          &plan->flops);
                     subh(proj4, acc4, SLOT_B, LB, tm.second, true, {SLOT_WS, T2, ub.size}, A.level, A.s, zt, kS,
-                         -1, 0, 1.0f, abFlag, &plan->flops);
+                         s.wr, SLOT_WIDTH, 1.0f, abFlag, &plan->flops);
                 }
             }
             proj3.finalize(); acc3.finalize(); proj4.finalize(); acc4.finalize();
@@ -883,7 +890,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 PhaseBuilder comb(cb);
                 for (const Sk& s : sk) {
                     const Block& C = g.blocks[s.b];
-                    addQr(q, q2, comb, s.Z, C.level, C.s, s.R);
+                    addQr(q, q2, comb, s.Z, C.level, C.s, s.R, s.wr);
                 }
                 comb.finalize();
             }
@@ -891,9 +898,10 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             for (const Sk& s : sk) {
                 const Block& C = g.blocks[s.b];
                 SvdTask v{};
-                v.roff = s.R; v.g1off = s.G1; v.g2off = s.G2; v.k = kS;
+                v.roff = s.R; v.g1off = s.G1; v.g2off = s.G2; v.k = kS; v.ld = kS;
                 v.cap = LC.cap[C.lrIndex];
                 v.rankIndex = C.lrIndex;
+                v.krank = s.wr; v.spare = oversample;
                 v.maxKeep = std::min(std::min(maxRank, v.cap), std::min(std::min(s.m, s.n), kS));
                 sv.svd.push_back(v);
                 plan->flops += 30.0 * kS * kS * kS;
@@ -911,12 +919,12 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                     for (int c = 0; c < ut.kChunks; ++c)
                         acc.add(ut.tileOfAtom[(size_t)(a - tb.atomBegin) * ut.kChunks + c],
                                 mkc(SLOT_WS, s.Y + (g.atomStart[a] - tb.off), s.m, SLOT_WS,
-                                    s.G1 + (int64_t)c * 64 * kS, kS, kS, -1, 0, formFlag, 1.0f));
+                                    s.G1 + (int64_t)c * 64 * kS, kS, kS, s.wr, SLOT_WIDTH, formFlag, 1.0f));
                 for (int a = sb.atomBegin; a < sb.atomEnd; ++a)
                     for (int c = 0; c < vt.kChunks; ++c)
                         acc.add(vt.tileOfAtom[(size_t)(a - sb.atomBegin) * vt.kChunks + c],
                                 mkc(SLOT_WS, s.Z + (g.atomStart[a] - sb.off), s.n, SLOT_WS,
-                                    s.G2 + (int64_t)c * 64 * kS, kS, kS, -1, 0, 0, 1.0f));
+                                    s.G2 + (int64_t)c * 64 * kS, kS, kS, s.wr, SLOT_WIDTH, 0, 1.0f));
                 plan->flops += 2.0 * (s.m + s.n) * kS * cap;
             }
             acc.finalize();

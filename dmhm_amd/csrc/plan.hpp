@@ -24,8 +24,11 @@
 
 namespace dmhm {
 
+// Slots 6 and 7 carry only integer arrays: per-block sketch widths (read
+// where a plan is width-indirect) and per-block flags of the a-posteriori
+// sketch check.
 enum Slot : uint8_t { SLOT_A = 0, SLOT_B = 1, SLOT_C = 2, SLOT_WS = 3, SLOT_X = 4, SLOT_Y = 5,
-                      NUM_SLOTS = 8 };
+                      SLOT_WIDTH = 6, SLOT_FLAG = 7, NUM_SLOTS = 8 };
 
 enum ContribFlags : uint8_t { CF_TRANS_L = 1, CF_TRANS_R = 2, CF_RUNTIME_ALPHA = 4, CF_RUNTIME_BETA = 8 };
 enum TileFlags : uint8_t { TF_ACCUMULATE = 1, TF_RUNTIME_BETA = 2 };
@@ -54,17 +57,21 @@ struct Tile {
 struct QrTask {              // Householder QR of an m x k panel, Q in place
     int64_t off;            // panel offset (slot)
     int64_t roff;           // R output (k x k upper triangle, ld rld) offset in the workspace, or -1
-    int32_t m, k;
+    int32_t m, k;           // k: static width (R rows written: k; columns: the effective width)
     int32_t ld, rld;        // panel and R leading dimensions
-    uint8_t slot, pad0, pad1, pad2;
-    int32_t pad;
+    uint8_t slot = 0, kslot = 0, pad1 = 0, pad2 = 0;
+    int32_t krank = -1;     // rank-array index of the effective width min(k, rank[kslot][krank]), or -1
 };
 
 struct SvdTask {             // SVD of R (k x k) -> G1 = V S (k x cap), G2 = U (k x cap)
-    int64_t roff, g1off, g2off;   // workspace offsets
+    int64_t roff, g1off, g2off;   // workspace offsets (R, G1, G2 with leading dimension ld)
     int32_t k, cap;
     int32_t rankIndex;      // rank slot entry written with the kept rank
     int32_t maxKeep;        // min(maxRank, m, n, k)
+    int32_t ld = 0;         // leading dimension of R, G1, G2 (the static width; 0: k)
+    int32_t krank = -1;     // effective width min(k, rank[SLOT_WIDTH][krank]), or -1
+    int32_t spare = 0;      // oversampling columns required below the tolerance (a-posteriori check)
+    int32_t pad = 0;
 };
 
 struct RngTask {
@@ -125,6 +132,11 @@ struct Phase {
     int maxMTile = 0;       // tallest tile
     int role = 0;           // 0 accumulation, 1 projection (profiling identity)
     double flops = 0;       // algorithmic flops (2 m q k per contribution at static sizes)
+    // Tile-engine autotuning state per execution slot (FP64, FP32, FP32 with
+    // BF16 operands allowed): candidates timed so far and the choice (-1 while
+    // tuning).
+    struct Tune { int choice = -1; int next = 0; float ms[8] = {0, 0, 0, 0, 0, 0, 0, 0}; };
+    Tune tune[3];
 };
 
 struct Plan {
@@ -217,6 +229,12 @@ public:
     // Whether the low-rank factors of A / B are expected at full rank, so that
     // the exact-factor terms they enter are stacked (see hhProduct).
     void setStacking(bool a, bool b) { stackA_ = a; stackB_ = b; }
+    // Width-indirect sketches: every admissible C block sketches with
+    // min(maxRank + oversample, widths[block]) columns, widths read at run time
+    // from the SLOT_WIDTH array (indexed like C's ranks); the SVD then flags
+    // (SLOT_FLAG array) blocks whose kept rank leaves fewer than `oversample`
+    // sketch columns below the truncation tolerance.
+    void setWidthIndirect(bool on) { widthIndirect_ = on; }
 
     // Sub-H product: out(|outBox| x k) (+)= op(M(b)) in(|inBox| x k). Leaves of
     // M inside block b contribute to the tiles of `out` (registered per atom in
@@ -240,6 +258,7 @@ private:
     int64_t wsBudget_ = int64_t(1) << 31;
     int tsqrRows_ = 0;
     bool stackA_ = true, stackB_ = true;
+    bool widthIndirect_ = false;
 };
 
 } // namespace dmhm

@@ -7,7 +7,11 @@ include/dmhm/hmat3d.hpp) and ``hmat_tools::EstimateTwoNorm``
 libdmhm_amd.so to the gfx950 kernels; nothing is computed in Python.
 
 Dense operands may be numpy arrays (host memory; copied through the library's
-staging buffers) or torch tensors on the GPU (passed by device pointer).
+staging buffers) or torch tensors on the GPU (passed by device pointer). torch
+bundles its own HIP runtime and the library binds whichever libamdhip64.so.7
+the process loaded first: torch tensors work when torch is imported before the
+library is loaded (both then share torch's runtime); imported afterwards, torch
+does not see the GPU.
 """
 import ctypes
 
@@ -267,10 +271,13 @@ class _HMatBase:
 
     AdjointMultiply = TransposeMultiply
 
-    def MultiplyHMat(self, alpha, B, beta=None, C=None, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1):
+    def MultiplyHMat(self, alpha, B, beta=None, C=None, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1,
+                     rank_model=None, rank_margin=1.0):
         """C := alpha A B (new matrix), or C := alpha A B + beta C in place when
-        ``C`` is given (MultiplyHMat-incl.hpp:12-380)."""
-        o = _lib.HHOpts(oversample, tol_mode, rel_tol, seed)
+        ``C`` is given (MultiplyHMat-incl.hpp:12-380). With ``rank_model`` (a
+        RankModel) and tol_mode=1 the sketch of every admissible block is
+        sized by the model's predicted rank + rank_margin + oversample."""
+        o = _hh_opts(oversample, tol_mode, rel_tol, seed, rank_model, rank_margin)
         if C is not None:
             check(_lib.load().dmhm_hmat_multiply_hmat_update(alpha, self._h, B._h, 0.0 if beta is None else beta,
                                                              C._h, ctypes.byref(o)))
@@ -282,7 +289,7 @@ class _HMatBase:
     def UpdateWith(self, alpha, B, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1):
         """A := A + alpha B with recompression of admissible blocks
         (src/HMat2d/HMat2d.cpp:746-784)."""
-        o = _lib.HHOpts(oversample, tol_mode, rel_tol, seed)
+        o = _hh_opts(oversample, tol_mode, rel_tol, seed)
         check(_lib.load().dmhm_hmat_update_with(self._h, alpha, B._h, ctypes.byref(o)))
         return self
 
@@ -323,10 +330,16 @@ class _HMatBase:
 
     def SchulzInvert(self, numIterations=-1, theta=1.5, confidence=6.0, stopTol=1e-4, maxIterations=30,
                      mixed=False, switchTol=1e-3, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1,
-                     raise_on_noconv=False, stall_stop=False):
-        """A := inv(A) by Schulz iteration (Invert-incl.hpp:128-196). Returns the report."""
+                     raise_on_noconv=False, stall_stop=False, graph=True, bf16=False, bf16Tol=0.5,
+                     rank_model=None, rank_margin=1.0):
+        """A := inv(A) by Schulz iteration (Invert-incl.hpp:128-196). Returns the
+        report. graph=True replays captured HIP graphs for the steady steps;
+        mixed=True runs FP32 steps until the estimate is below switchTol, with
+        BF16 tile operands while it is at least bf16Tol when bf16=True;
+        rank_model (with tol_mode=1) sizes every product's sketches."""
         o = _lib.SchulzOpts(numIterations, maxIterations, theta, confidence, stopTol, 1 if mixed else 0,
-                            switchTol, _lib.HHOpts(oversample, tol_mode, rel_tol, seed), 1 if stall_stop else 0)
+                            switchTol, _hh_opts(oversample, tol_mode, rel_tol, seed, rank_model, rank_margin),
+                            1 if stall_stop else 0, 1 if graph else 0, 1 if bf16 else 0, bf16Tol)
         rep = _lib.SchulzReport()
         code = _lib.load().dmhm_hmat_schulz_invert(self._h, ctypes.byref(o), ctypes.byref(rep))
         if code == 4 and not raise_on_noconv:
@@ -346,7 +359,7 @@ class _HMatBase:
         ms = np.zeros(6)
         fl = np.zeros(2)
         dp = ctypes.POINTER(ctypes.c_double)
-        o = _lib.HHOpts(oversample, tol_mode, rel_tol, seed)
+        o = _hh_opts(oversample, tol_mode, rel_tol, seed)
         check(_lib.load().dmhm_profile_hh(alpha, self._h, B._h, ctypes.byref(o), ms.ctypes.data_as(dp),
                                           fl.ctypes.data_as(dp)))
         return ms, fl
@@ -365,11 +378,102 @@ class _HMatBase:
         n = min(cnt.value, cap)
         return [(int(kinds[i]), float(ms[i]), float(fl[i])) for i in range(n)]
 
+    def rank_features(self):
+        """Features (numLowRank x DMHM_RANK_FEATURES, float32) of the admissible
+        blocks of a product whose left operand is this matrix: the input of
+        the rank model in that product."""
+        n = int(self.info().numLowRank)
+        out = np.zeros((n, _lib.DMHM_RANK_FEATURES), dtype=np.float32)
+        check(_lib.load().dmhm_hmat_rank_features(self._h, out.ctypes.data_as(_lib._fp), out.size))
+        return out
+
     def to_dense(self):
         n = self.Height()
         out = np.zeros((n, n), order="F")
         check(_lib.load().dmhm_hmat_to_dense(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n))
         return out
+
+
+def _hh_opts(oversample, tol_mode, rel_tol, seed, rank_model=None, rank_margin=1.0):
+    return _lib.HHOpts(oversample, tol_mode, rel_tol, seed, rank_model._h if rank_model is not None else None,
+                       rank_margin)
+
+
+def last_product_stats():
+    """The last product that used a rank model: dict of admissible blocks, sum
+    of the predicted sketch widths, blocks recomputed at full width, and
+    blocks x full width."""
+    out = np.zeros(4, dtype=np.int64)
+    check(_lib.load().dmhm_last_product_stats(out.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+    return dict(zip(["blocks", "width_sum", "fallbacks", "full_width_sum"], out.tolist()))
+
+
+class RankModel:
+    """The learned rank model (3-layer perceptron over DMHM_RANK_FEATURES block
+    features, two hidden layers of 64 ReLU units), trained and evaluated by the
+    library's GPU kernels."""
+
+    def __init__(self, seed=0):
+        _lib.init()
+        h = ctypes.c_void_p()
+        check(_lib.load().dmhm_rank_model_create(seed, ctypes.byref(h)))
+        self._h = h
+        self.num_params = _lib.load().dmhm_rank_model_num_params()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.dmhm_rank_model_destroy(h)
+            self._h = None
+
+    @property
+    def params(self):
+        p = np.zeros(self.num_params, dtype=np.float32)
+        check(_lib.load().dmhm_rank_model_get_params(self._h, p.ctypes.data_as(_lib._fp), p.size))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(_lib.load().dmhm_rank_model_set_params(self._h, p.ctypes.data_as(_lib._fp), p.size))
+
+    @staticmethod
+    def _xt(X, t=None):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2 or X.shape[1] != _lib.DMHM_RANK_FEATURES:
+            raise ValueError("X must be n x %d" % _lib.DMHM_RANK_FEATURES)
+        if t is None:
+            return X, None
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        if t.shape != (X.shape[0],):
+            raise ValueError("t must have one target per row of X")
+        return X, t
+
+    def gradient(self, X, t):
+        """(loss, dloss/dparams) of mean (y - t)^2 on the GPU."""
+        X, t = self._xt(X, t)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        check(_lib.load().dmhm_rank_model_gradient(self._h, X.ctypes.data_as(_lib._fp), t.ctypes.data_as(_lib._fp),
+                                                   X.shape[0], g.ctypes.data_as(_lib._fp),
+                                                   loss.ctypes.data_as(_lib._fp)))
+        return float(loss[0]), g
+
+    def predict(self, X):
+        X, _ = self._xt(X)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        check(_lib.load().dmhm_rank_model_predict(self._h, X.ctypes.data_as(_lib._fp), X.shape[0],
+                                                  y.ctypes.data_as(_lib._fp)))
+        return y
+
+    def train(self, X, t, steps, batch=1024, lr=1e-3, seed=0):
+        """Adam on the GPU; returns the per-step mini-batch losses."""
+        X, t = self._xt(X, t)
+        losses = np.zeros(max(steps, 1), dtype=np.float32)
+        check(_lib.load().dmhm_rank_model_train(self._h, X.ctypes.data_as(_lib._fp), t.ctypes.data_as(_lib._fp),
+                                                X.shape[0], steps, batch, lr, seed,
+                                                losses.ctypes.data_as(_lib._fp)))
+        return losses[:steps]
 
 
 class HMat2d(_HMatBase):

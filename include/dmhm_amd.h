@@ -127,11 +127,27 @@ int dmhm_hmat_set_to_random(dmhm_hmat h, uint64_t seed);
 int dmhm_hmat_multiply_dense(dmhm_hmat h, char op, double alpha, const void* X, int ldx, int k,
                              double beta, void* Y, int ldy, int memory);
 
+/* Rank model: a 3-layer perceptron (DMHM_RANK_FEATURES inputs, two hidden
+ * layers of 64 ReLU units, one linear output) predicting the numerical rank of
+ * an admissible block of a product from the block's features (level, box
+ * sizes and offsets, coefficient statistics of the operator, the block's rank
+ * in the left operand). No counterpart in the reference, whose sketch width is
+ * fixed at SampleRank(maxRank) = maxRank + Oversample()
+ * (include/dmhm/hmat2d.hpp:32-33). */
+typedef struct dmhm_rank_model_s* dmhm_rank_model;
+#define DMHM_RANK_FEATURES 12
+
 typedef struct {
     int oversample;      /* sketch width = maxRank + oversample (Oversample(), global.cpp:30) */
     int tolMode;         /* 0: fixed rank maxRank (reference); 1: keep s_i > relTol*s_0      */
     double relTol;       /* relative tolerance for tolMode 1 (Truncation-incl.hpp:37-63)     */
     uint64_t seed;       /* sketch seed                                                      */
+    dmhm_rank_model rankModel; /* tolMode 1 only (else ignored): per-block sketch width =
+                            predicted rank + rankMargin + oversample (at most maxRank +
+                            oversample); blocks whose SVD keeps more than width -
+                            oversample values are recomputed at full width. NULL: the
+                            fixed width maxRank + oversample                              */
+    double rankMargin;
 } dmhm_hh_opts;
 
 /* C := alpha A B, HMat2d::Multiply(alpha, B, C) (src/HMat2d/MultiplyHMat-incl.hpp:12-187).
@@ -206,6 +222,15 @@ typedef struct {
     dmhm_hh_opts hh;
     int stallStop;       /* 1: also stop (or leave low precision) once the estimate
                             is below 0.1 and no longer halves over two iterations */
+    int graph;           /* 1: once the plans of a precision are built and their
+                            tile engines tuned by eager steps, run each step as a
+                            replay of a captured HIP graph (same launches, same
+                            seeds; one readback per step for the stop decision) */
+    int bf16;            /* mixed mode: the tile GEMMs of the FP32 steps take BF16
+                            operands (FP32 accumulation) while the residual
+                            estimate is >= bf16Tol; QR, SVD and the estimate stay
+                            in FP64 arithmetic */
+    double bf16Tol;
 } dmhm_schulz_opts;
 
 typedef struct {
@@ -215,11 +240,15 @@ typedef struct {
     double normEstimate;         /* estimate of ||A||_2 used for X_0 */
     double seconds;              /* wall time of the whole inversion */
     double history[128];         /* residual estimate per iteration */
+    int graphIterations;         /* iterations run as HIP graph replays */
+    int iterationsBf16;          /* iterations with BF16 tile operands */
 } dmhm_schulz_report;
 
 /* A := inv(A) by Schulz iteration X_{k+1} = (2I - X_k A) X_k from
  * X_0 = alpha A^T, alpha = 2/est^2 (HMat2d::SchulzInvert,
- * src/HMat2d/Invert-incl.hpp:128-196). Replaces h's contents. */
+ * src/HMat2d/Invert-incl.hpp:128-196). Replaces h's contents. The iterates
+ * ping-pong between two preallocated buffers; no allocation after the first
+ * two steps of each precision. */
 int dmhm_hmat_schulz_invert(dmhm_hmat h, const dmhm_schulz_opts* opts, dmhm_schulz_report* report);
 
 /* Packed byte image of an H-matrix (HMat2d::PackedSize / Pack / Unpack and
@@ -260,6 +289,43 @@ int dmhm_profile_matvec(dmhm_hmat h, int k, int reps, double* ms, double* bytes)
  *   launches, flops[1] of the longest tile launch. */
 int dmhm_profile_hh(double alpha, dmhm_hmat A, dmhm_hmat B, const dmhm_hh_opts* opts, double* ms,
                     double* flops);
+
+/* Rank model (no counterpart in the reference; see dmhm_hh_opts). Created
+ * with He-initialised weights from `seed`; parameters are FP32 on the device
+ * (dmhm_rank_model_num_params floats, layout W1[NF x 64], b1[64], W2[64 x 64],
+ * b2[64], w3[64], b3). X: n x DMHM_RANK_FEATURES row-major host array, t: n
+ * target ranks. gradient: d/dparams of mean (y - t)^2 (host output) and the
+ * loss; train: `steps` Adam steps (learning rate lr) over consecutive
+ * mini-batches of `batch` samples of a seeded shuffle of the data, on the GPU;
+ * losses (if non-null) receives each step's mini-batch loss. */
+int dmhm_rank_model_create(uint64_t seed, dmhm_rank_model* out);
+int dmhm_rank_model_destroy(dmhm_rank_model m);
+int dmhm_rank_model_num_params(void);
+int dmhm_rank_model_get_params(dmhm_rank_model m, float* params, int count);
+int dmhm_rank_model_set_params(dmhm_rank_model m, const float* params, int count);
+int dmhm_rank_model_gradient(dmhm_rank_model m, const float* X, const float* t, int n, float* grads,
+                             float* loss);
+int dmhm_rank_model_predict(dmhm_rank_model m, const float* X, int n, float* y);
+int dmhm_rank_model_train(dmhm_rank_model m, const float* X, const float* t, int n, int steps, int batch,
+                          float lr, uint64_t seed, float* losses);
+/* Features (count = numLowRank x DMHM_RANK_FEATURES floats, host) of the
+ * admissible blocks of a product whose left operand is A, in the order of
+ * dmhm_hmat_ranks: the input the rank model sees in that product. */
+int dmhm_hmat_rank_features(dmhm_hmat A, float* out, int count);
+/* The last product with a rank model: out[0] admissible blocks, out[1] sum of
+ * the predicted sketch widths, out[2] blocks recomputed at full width,
+ * out[3] blocks x full width. */
+int dmhm_last_product_stats(long long* out);
+
+/* Tile engine of the grouped small GEMMs (measurement and test support, no
+ * counterpart in the reference): -1 autotunes every tile phase (each phase
+ * times the candidate engines on its first runs and keeps the fastest; the
+ * default), 0 the LDS-staged vector-FMA kernel, 1 the MFMA kernel, 2 the
+ * MFMA kernel with one-chunk prefetch, 3 / 4 the same with BF16 operands
+ * where BF16 is allowed (the FP32 steps of a mixed Schulz inversion).
+ * DMHM_TILE_ENGINE=auto|lds|mfma|mfma_pf|bf16|bf16_pf sets it at dmhm_init. */
+int dmhm_set_tile_engine(int engine);
+int dmhm_get_tile_engine(void);
 
 /* Per-phase timings of the last dmhm_profile_hh call (measurement support,
  * no counterpart in the reference): phase kind (0 tile, 1 QR, 2 SVD, 3 RNG),

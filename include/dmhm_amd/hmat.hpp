@@ -283,6 +283,9 @@ public:
         o.switchTol = 1e-3;
         o.hh = hh_options();
         o.stallStop = 0;
+        o.graph = 1;
+        o.bf16 = 0;
+        o.bf16Tol = 0.5;
         return o;
     }
     const dmhm_schulz_report& LastSchulzReport() const { return report_; }
@@ -338,6 +341,8 @@ protected:
         o.tolMode = 0;
         o.relTol = 1e-10;
         o.seed = next_seed();
+        o.rankModel = nullptr;
+        o.rankMargin = 0;
         return o;
     }
     template <typename F> void replace(F make)

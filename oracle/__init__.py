@@ -96,6 +96,7 @@ def _bind(L):
             "orc_randomized_multiply": (None, [ctypes.c_int, ctypes.c_double, _vp, _vp, ctypes.c_int,
                                                _dp, _dp, _ip]),
             "orc_hmat_multiply_child_row": (_vp, [ctypes.c_double, _vp, _vp, ctypes.c_int]),
+            "orc_hmat_multiply_child": (_vp, [ctypes.c_double, _vp, _vp, ctypes.c_int, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -276,6 +277,10 @@ class HMat:
 
     def matmul(self, other, alpha=1.0):
         return HMat(self.L.orc_hmat_multiply(alpha, self.ptr, other.ptr), self.L)
+
+    def matmul_child(self, other, t, s, alpha=1.0):
+        """Child (t, s) of alpha self other only (a timing sample)."""
+        return HMat(self.L.orc_hmat_multiply_child(alpha, self.ptr, other.ptr, t, s), self.L)
 
     def matmul_child_row(self, other, t, alpha=1.0):
         """Children (t, *) of alpha self other only (a timing sample)."""

@@ -78,6 +78,14 @@ void* orc_hmat_create(int dim, const int* sizes, int numLevels, int maxRank, int
     return h.release();
 }
 void orc_hmat_free(void* h) { delete static_cast<HMat*>(h); }
+// Child C(t, s) of C := alpha A B (timing sample of a product).
+void* orc_hmat_multiply_child(double alpha, void* A, void* B, int t, int s)
+{
+    auto C = std::make_unique<HMat>();
+    C->type = HMat::LOW_RANK;
+    HMat::multiply_child(alpha, *static_cast<HMat*>(A), *static_cast<HMat*>(B), *C, t, s);
+    return C.release();
+}
 // Children C(t, *) of C := alpha A B (timing sample of a product).
 void* orc_hmat_multiply_child_row(double alpha, void* A, void* B, int t)
 {

@@ -596,22 +596,29 @@ void HMat::multiply(double alpha, const HMat& A, const HMat& B, HMat& C)
     }
 }
 
-void HMat::multiply_child_row(double alpha, const HMat& A, const HMat& B, HMat& C, int t)
+void HMat::multiply_child(double alpha, const HMat& A, const HMat& B, HMat& C, int t, int s)
 {
     if (A.width() != B.height() || A.numLevels != B.numLevels)
         throw std::logic_error("Attempted nonconformal matrix-matrix multiply");
-    set_product_geometry(A, B, C);
-    C.ch.clear(); C.F = LowRank(); C.D = Dense();
-    if (A.type != NODE || B.type != NODE || C.admissible() || C.numLevels <= 1)
-        throw std::logic_error("multiply_child_row needs NODE operands and a NODE product");
-    C.type = NODE;
-    make_children(C);
-    const int nc = C.nc();
-    for (int s = 0; s < nc; ++s) {
-        multiply(alpha, A.child(t, 0), B.child(0, s), C.child(t, s));
-        for (int u = 1; u < nc; ++u)
-            multiply_update(alpha, A.child(t, u), B.child(u, s), 1.0, C.child(t, s));
+    if (C.type != NODE || C.ch.empty()) {
+        set_product_geometry(A, B, C);
+        C.ch.clear(); C.F = LowRank(); C.D = Dense();
+        if (A.type != NODE || B.type != NODE || C.admissible() || C.numLevels <= 1)
+            throw std::logic_error("a child product needs NODE operands and a NODE product");
+        C.type = NODE;
+        make_children(C);
     }
+    const int nc = C.nc();
+    multiply(alpha, A.child(t, 0), B.child(0, s), C.child(t, s));
+    for (int u = 1; u < nc; ++u)
+        multiply_update(alpha, A.child(t, u), B.child(u, s), 1.0, C.child(t, s));
+}
+
+void HMat::multiply_child_row(double alpha, const HMat& A, const HMat& B, HMat& C, int t)
+{
+    C.ch.clear();
+    C.type = LOW_RANK;                  // forces multiply_child to set up C's children
+    for (int s = 0; s < (1 << A.dim); ++s) multiply_child(alpha, A, B, C, t, s);
 }
 
 void HMat::multiply_update(double alpha, const HMat& A, const HMat& B, double beta, HMat& C)

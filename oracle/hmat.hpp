@@ -98,6 +98,8 @@ public:
     // with a NODE product (the rest of C stays empty): a bounded sample of the
     // product's work, used to time the CPU baseline.
     static void multiply_child_row(double alpha, const HMat& A, const HMat& B, HMat& C, int t);
+    // Only the child C(t, s) of C := alpha A B (same conditions; timing sample).
+    static void multiply_child(double alpha, const HMat& A, const HMat& B, HMat& C, int t, int s);
 
     // Schulz inversion of *this (Invert-incl.hpp:128-196). history receives the
     // per-iteration residual estimates; returns the number of iterations run.

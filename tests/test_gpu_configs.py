@@ -145,3 +145,30 @@ def test_batch_members_schulz_inverse(side):
     # measured: side 16 dense 1.7e-7 / estimate 1.4e-7, side 20 dense 9.6e-6 / estimate 5.5e-6
     assert res < 1e-4
     assert res / 3 <= rep.residualEstimate <= 3 * res
+
+
+def test_c2_exact_product_matches_oracle():
+    """2D 256^2 lognormal operator (numLevels 6, maxRank 16): the product A A
+    of the imported operator on the GPU against the FP64 oracle's product of
+    the same operator, applied to seeded right-hand sides (strong
+    admissibility leaves A^2 without admissible couplings, so both products
+    are exact up to roundoff; the fixed-rank oracle keeps roundoff-size
+    rank-16 factors there, as the reference does), and against A (A X) from
+    the CSR; FP64, 1e-12 relative."""
+    import oracle as orc
+    cfg = _config("c2")
+    args = (cfg["rowptr"], cfg["colind"], cfg["vals"])
+    n = int(np.prod(cfg["sizes"]))
+    log("c2 exact product: GPU")
+    A = dmhm_amd.HMat2d.from_csr(cfg["sizes"], cfg["num_levels"], cfg["max_rank"], True, *args)
+    C = A.Multiply(1.0, A)
+    log("c2 exact product: oracle")
+    R = orc.HMat.from_csr(2, cfg["sizes"], cfg["num_levels"], cfg["max_rank"], True, *args)
+    Cr = R.matmul(R)
+    X = np.random.default_rng(5).standard_normal((n, 4))
+    y_gpu = C.Multiply(1.0, X)
+    y_orc = Cr.multiply(X)
+    S = _csr(cfg)
+    y_csr = S @ (S @ X)
+    assert np.abs(y_gpu - y_orc).max() <= 1e-12 * np.abs(y_orc).max()
+    assert np.abs(y_gpu - y_csr).max() <= 1e-12 * np.abs(y_csr).max()

@@ -88,35 +88,6 @@ def test_matvec_fp32():
     assert _relmax(H.Multiply(1.0, X), R.multiply(X)) < 1e-5
 
 
-def test_matvec_device_pointers():
-    torch = pytest.importorskip("torch")
-    if not torch.cuda.is_available():
-        pytest.skip("torch sees no GPU")
-    op = W.small_operator(16, 3)
-    H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 4, True, *_args(op))
-    X = np.random.default_rng(0).standard_normal((256, 3))
-    Xd = torch.from_numpy(X.T.copy()).cuda().t()             # column-major view
-    Yd = H.Multiply(1.0, Xd)
-    torch.cuda.synchronize()
-    assert _relmax(Yd.cpu().numpy(), H.Multiply(1.0, X)) < 1e-14
-
-
-def test_matvec_device_operands_from_torch_kernels():
-    """X produced by a torch kernel on the device and Y freshly allocated by
-    the wrapper: the library stream must see both finished."""
-    torch = pytest.importorskip("torch")
-    if not torch.cuda.is_available():
-        pytest.skip("torch sees no GPU")
-    op = W.small_operator(16, 3)
-    H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 4, True, *_args(op))
-    g = torch.Generator(device="cuda").manual_seed(5)
-    for k in (1, 3):
-        Xd = torch.randn((k, 256), dtype=torch.float64, device="cuda", generator=g).t()
-        Xd = Xd * 3.0 + 1.0                        # a second pending kernel on torch's stream
-        Yd = H.Multiply(1.0, Xd)
-        assert _relmax(Yd.cpu().numpy(), H.Multiply(1.0, Xd.cpu().numpy())) < 1e-14
-
-
 def test_multiply_writes_numpy_y_in_place():
     """Multiply(alpha, X, beta, Y) updates Y in place also when Y is C-ordered
     or of another dtype (the wrapper copies and writes back)."""
@@ -276,6 +247,70 @@ def test_schulz_exact_rank_reaches_roundoff():
     res = np.linalg.norm(np.eye(256) - D @ H.to_dense(), 2)
     assert rep.residualEstimate < 1e-9
     assert res < 1e-9
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_schulz_graph_replay_matches_eager(mixed):
+    """Steps replayed from captured HIP graphs launch what the eager steps
+    launch (same plans, same counter-offset seeds): every step's residual
+    estimate and the final inverse agree. Each replay is followed by the
+    step's readback and host decision, so replays interleave with eager work."""
+    op = W.small_operator(32, 3, smooth=False)
+    kw = dict(numIterations=-1, stopTol=1e-13, maxIterations=14, mixed=mixed, switchTol=1e-2, seed=7)
+    # One fixed tile engine for both runs (autotuning would time a different
+    # candidate engine in each run's first steps).
+    prev = dmhm_amd.set_tile_engine("mfma")
+    try:
+        He = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 8, True, *_args(op))
+        Hg = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 8, True, *_args(op))
+        re = He.SchulzInvert(graph=False, **kw)
+        rg = Hg.SchulzInvert(graph=True, **kw)
+    finally:
+        dmhm_amd.set_tile_engine(prev)
+    assert re.graphIterations == 0
+    assert rg.graphIterations >= 8
+    assert re.iterations == rg.iterations == 14
+    assert re.iterationsLowPrecision == rg.iterationsLowPrecision
+    he = np.array(re.history[:re.iterations])
+    hg = np.array(rg.history[:rg.iterations])
+    assert np.all(np.abs(he - hg) <= 1e-9 * np.abs(he))
+    De, Dg = He.to_dense(), Hg.to_dense()
+    assert _relmax(Dg, De) < 1e-12
+    # 14 steps leave this operator in the linear phase: I - A X contracts
+    # (spectral norm below 1) and is the same for both runs.
+    D = W.csr_to_dense(*_args(op), 1024)
+    rg_norm = np.linalg.norm(np.eye(1024) - D @ Dg, 2)
+    assert rg_norm < 1.0
+    assert abs(rg_norm - np.linalg.norm(np.eye(1024) - D @ De, 2)) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_every_tile_engine_gives_the_same_product(prec):
+    """The LDS kernel, the MFMA kernel and its prefetching form (and the BF16
+    forms, which fall back to FP32/FP64 MFMA outside a mixed Schulz loop)
+    compute the same exact product: rank-16 operands on a 20 x 12 grid, whose
+    level-2 boxes hold 15 points, so no block needs more than rank 16; against
+    the dense product (FP64 1e-12, FP32 1e-5)."""
+    op = W.rect_operator([20, 12], 3)      # level-2 boxes of 15 points: rank-16 products are exact
+    sizes = op["sizes"]
+    n = int(np.prod(sizes))
+    p = dmhm_amd.DMHM_F64 if prec == "f64" else dmhm_amd.DMHM_F32
+    A = dmhm_amd.HMat2d.from_csr(sizes, 3, 16, True, *_args(op))
+    A.SetToRandom(seed=3)
+    B = dmhm_amd.HMat2d.from_csr(sizes, 3, 16, True, *_args(op))
+    B.SetToRandom(seed=4)
+    ref = A.to_dense() @ B.to_dense()
+    if p == dmhm_amd.DMHM_F32:
+        A, B = A.Convert(p), B.Convert(p)
+    tol = 1e-12 if p == dmhm_amd.DMHM_F64 else 1e-5
+    prev = dmhm_amd.set_tile_engine("lds")
+    try:
+        for eng in ("lds", "mfma", "mfma_pf", "bf16", "bf16_pf", "auto", "auto", "auto", "auto"):
+            dmhm_amd.set_tile_engine(eng)
+            C = A.Multiply(0.5, B, oversample=4)
+            assert _relmax(C.to_dense(), 0.5 * ref) < tol, eng
+    finally:
+        dmhm_amd.set_tile_engine(prev)
 
 
 def _inverse_floor(D, leaves, rank):

@@ -122,6 +122,38 @@ def test_hh_product_exact_when_rank_covers_blocks(n, L, strong, rank):
     assert np.linalg.norm(C.to_dense() - ref) <= 1e-11 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("alpha,beta", [(-0.5, 0.25), (1.0, -2.0), (2.0, 0.0)])
+def test_hh_update_form_matches_dense_closed_form(alpha, beta):
+    """C := alpha A B + beta C (MultiplyHMat-incl.hpp:189-380) with maxRank at
+    least every admissible block's size (16 x 16 grid, 3 levels: admissible
+    blocks are at most 16 x 16): every RoundedUpdate and sketch is exact, so
+    the update form must equal alpha A B + beta C computed densely, for beta
+    other than 1 as well."""
+    op, N, args = _op(16, 3)
+    orc.rng_initialize()
+    A = orc.HMat.from_csr(2, op["sizes"], 3, 16, True, *args)
+    A.set_to_random()
+    B = orc.HMat.from_csr(2, op["sizes"], 3, 16, True, *args)
+    B.set_to_random()
+    C = orc.HMat.from_csr(2, op["sizes"], 3, 16, True, *args)
+    C.set_to_random()
+    ref = alpha * A.to_dense() @ B.to_dense() + beta * C.to_dense()
+    C.multiply_update(alpha, A, B, beta)
+    assert np.linalg.norm(C.to_dense() - ref) <= 1e-11 * np.linalg.norm(ref)
+
+
+def test_hh_update_form_sparse_operands_exact():
+    """Imported operands (rank-0 admissible blocks, exact rank-1 products
+    below the cap): C := alpha A A + beta A equals the dense closed form."""
+    op, N, args = _op(32, 3)
+    D = W.csr_to_dense(*args, N)
+    A = orc.HMat.from_csr(2, op["sizes"], 3, 8, True, *args)
+    C = orc.HMat.from_csr(2, op["sizes"], 3, 8, True, *args)
+    C.multiply_update(0.75, A, A, -3.0)
+    ref = 0.75 * D @ D - 3.0 * D
+    np.testing.assert_allclose(C.to_dense(), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
 def test_hh_product_sparse_operand_exact():
     op, N, args = _op(32, 3)
     A = orc.HMat.from_csr(2, op["sizes"], 3, 8, True, *args)
