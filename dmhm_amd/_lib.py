@@ -93,6 +93,9 @@ SIGNATURES = {
     "dmhm_hmat_multiply_dense": (ctypes.c_int, [_h, ctypes.c_char, ctypes.c_double, ctypes.c_void_p,
                                                 ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
                                                 ctypes.c_int, ctypes.c_int]),
+    "dmhm_hmat_multiply_dense_noalloc": (ctypes.c_int, [_h, ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
+                                                        ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                                        ctypes.c_int]),
     "dmhm_hmat_multiply_hmat": (ctypes.c_int, [ctypes.c_double, _h, _h, _hp, ctypes.POINTER(HHOpts)]),
     "dmhm_hmat_multiply_hmat_update": (ctypes.c_int, [ctypes.c_double, _h, _h, ctypes.c_double, _h,
                                                       ctypes.POINTER(HHOpts)]),

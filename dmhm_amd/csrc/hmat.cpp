@@ -351,6 +351,35 @@ static void multiply_dense(dmhm_hmat_s* h, bool trans, double alpha, const void*
     else run_plan<float>(p, b, (float)alpha, (float)beta, {0, 0}, 0);
 }
 
+// H*X (k <= 8, device operands) on the streaming plan with the projections
+// in the device scratch array: no device allocation, no host staging. The
+// plan's index tables must exist already (built by a multiply_dense call of
+// the same shape), since building them allocates device memory.
+static void multiply_dense_noalloc(dmhm_hmat_s* h, double alpha, const void* X, int ldx, int k, double beta,
+                                   void* Y, int ldy)
+{
+    if (k < 1 || k > 8) throw Error(DMHM_EINVAL, "the allocation-free H*X takes 1 to 8 right-hand sides");
+    GeoCtx& gc = *h->gc;
+    const std::string key = "mvs:" + std::to_string(h->lay->id) + ":0:" + std::to_string(k) + ":" +
+                            std::to_string(ldx) + ":" + std::to_string(ldy);
+    auto it = gc.plans.find(key);
+    if (it == gc.plans.end())
+        throw Error(DMHM_EINVAL, "no prepared plan for this shape: call dmhm_hmat_multiply_dense once with op 'N' "
+                                 "and the same k, ldx and ldy first (that call may allocate)");
+    Plan& p = *it->second;
+    size_t cap = 0;
+    void* ws = scratch_address(g_lib.device, &cap);
+    if (!ws) throw Error(DMHM_ENODEV, "device scratch is not set up (dmhm_init)");
+    const size_t need = (size_t)p.wsElems * h->esize();
+    if (need > cap)
+        throw Error(DMHM_EINVAL, "workspace of " + std::to_string(need) + " bytes exceeds the scratch maximum of " +
+                                 std::to_string(cap) + " bytes");
+    Bases b = bases_for(h, nullptr, nullptr, const_cast<void*>(X), Y);
+    b.val[SLOT_WS] = ws;
+    if (h->prec == DMHM_F64) run_plan<double>(p, b, alpha, beta, {0, 0}, 0);
+    else run_plan<float>(p, b, (float)alpha, (float)beta, {0, 0}, 0);
+}
+
 // Static features of every admissible block of a geometry (kStaticFeatures
 // floats each, on the device, built once per operator statistics): level;
 // log sizes of the target and source boxes; log of their distance in boxes;
@@ -1468,6 +1497,7 @@ int dmhm_init(int device)
         HIPCHECK(hipMalloc((void**)&g_lib.dScalars, 16 * sizeof(double)));
         HIPCHECK(hipMalloc((void**)&g_lib.dCounter, sizeof(uint64_t)));
         HIPCHECK(hipHostMalloc((void**)&g_lib.hReadback, 4 * sizeof(double), hipHostMallocDefault));
+        scratch_setup(device);
         if (const char* e = getenv("DMHM_TILE_ENGINE")) {
             const std::string v(e);
             if (v == "lds") g_lib.tileEngine = TILE_LDS;
@@ -1839,6 +1869,19 @@ int dmhm_set_tile_engine(int engine)
 }
 
 int dmhm_get_tile_engine(void) { return g_lib.tileEngine; }
+
+int dmhm_hmat_multiply_dense_noalloc(dmhm_hmat h, double alpha, const void* X, int ldx, int k, double beta,
+                                     void* Y, int ldy)
+{
+    return guard([&] {
+        require_ready();
+        if (!h || !X || !Y) throw Error(DMHM_EINVAL, "null pointer");
+        const int n = h->geo().n;
+        if (ldx < n || ldy < n) throw Error(DMHM_EINVAL, "bad leading dimension");
+        multiply_dense_noalloc(h, alpha, X, ldx, k, beta, Y, ldy);
+        HIPCHECK(hipStreamSynchronize(g_lib.stream));
+    });
+}
 
 int dmhm_hmat_multiply_hmat(double alpha, dmhm_hmat A, dmhm_hmat B, dmhm_hmat* C, const dmhm_hh_opts* opts)
 {

@@ -85,4 +85,10 @@ void launch_rank_features(const float* stat, int nstat, const int* prev, float i
                           hipStream_t s);
 void launch_widen_flagged(int* widths, int* flags, int n, int wmax, int* count, long long* wsum, hipStream_t s);
 
+// Device scratch of the allocation-free H*X (noalloc.hip): scratch_setup takes
+// the array's address on `device` (which must be current) once;
+// scratch_address returns it (nullptr before set-up) and its size in bytes.
+void scratch_setup(int device);
+void* scratch_address(int device, size_t* bytes);
+
 } // namespace dmhm

@@ -127,6 +127,17 @@ int dmhm_hmat_set_to_random(dmhm_hmat h, uint64_t seed);
 int dmhm_hmat_multiply_dense(dmhm_hmat h, char op, double alpha, const void* X, int ldx, int k,
                              double beta, void* Y, int ldy, int memory);
 
+/* Y := alpha H X + beta Y for device X, Y (n x k, 1 <= k <= 8) without any
+ * device allocation: no counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops). The
+ * projections P = V^T X live in a 64 MiB device array per GPU held by the
+ * library until the process ends; a workspace larger than that is rejected
+ * (DMHM_EINVAL). The plan must have been built by one dmhm_hmat_multiply_dense
+ * call with op 'N' and the same k, ldx, ldy (else DMHM_EINVAL). Calls that
+ * overlap on one GPU race on the shared array. Returns after the GPU work. */
+int dmhm_hmat_multiply_dense_noalloc(dmhm_hmat h, double alpha, const void* X, int ldx, int k,
+                                     double beta, void* Y, int ldy);
+
 /* Rank model: a 3-layer perceptron (DMHM_RANK_FEATURES inputs, two hidden
  * layers of 64 ReLU units, one linear output) predicting the numerical rank of
  * an admissible block of a product from the block's features (level, box
