@@ -8,6 +8,7 @@ from ._lib import (DMHM_F64, DMHM_F32, DmhmError, NonConformalError, NotConverge
                    init, load, set_tile_engine)
 from .hmat import (HMat2d, HMat3d, EstimateTwoNorm, RoundedUpdate, Compress, RankModel,  # noqa: F401
                    last_product_stats,
-                   build_natural_to_hierarchical_map, geometry_block_counts, geometry_product_terms,
+                   build_natural_to_hierarchical_map, build_natural_to_hierarchical_map_device,
+                   geometry_block_counts, geometry_product_terms,
                    geometry_leaves)
 from . import workloads  # noqa: F401

@@ -77,6 +77,7 @@ SIGNATURES = {
     "dmhm_last_error": (ctypes.c_char_p, []),
     "dmhm_synchronize": (ctypes.c_int, []),
     "dmhm_build_natural_to_hierarchical_map": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _ip]),
+    "dmhm_build_natural_to_hierarchical_map_device": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _ip]),
     "dmhm_geometry_block_counts": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _llp, _llp,
                                                   ctypes.c_int]),
     "dmhm_geometry_leaves": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int,

@@ -118,11 +118,70 @@ struct GeoCtx {
     // diagonal dense leaves (t == s) for AddConstantToDiagonal, per layout id
     std::map<int, std::pair<int64_t*, int*>> diagTables;
     std::map<int, int> diagCount;
+    // Device copy of the block tree (CSR import, index map), built on first use.
+    DeviceBuffer devTree;
+    GeoDev dev;
+    bool devReady = false;
     GeoCtx(int dim, const int* sizes, int L, bool strong) : geo(dim, sizes, L, strong) {}
     ~GeoCtx() {
         for (auto& kv : diagTables) { (void)hipFree(kv.second.first); (void)hipFree(kv.second.second); }
+        devTree.release();
     }
 };
+
+static hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+
+// Uploads the block tree of a geometry as the flat int arrays of GeoDev (one
+// allocation), once.
+static const GeoDev& geo_dev(GeoCtx& gc)
+{
+    if (gc.devReady) return gc.dev;
+    const Geometry& g = gc.geo;
+    const int nc = g.nc();
+    std::vector<int> lvlBase, boxOff, boxSize, boxChild;
+    int base = 0;
+    for (int l = 0; l < g.numLevels; ++l) {
+        lvlBase.push_back(base);
+        base += (int)g.boxes[l].size();
+    }
+    for (int l = 0; l < g.numLevels; ++l)
+        for (const Box& B : g.boxes[l]) {
+            boxOff.push_back(B.off);
+            boxSize.push_back(B.size);
+            for (int c = 0; c < nc; ++c)
+                boxChild.push_back(l + 1 < g.numLevels ? lvlBase[l + 1] + B.child[c] : -1);
+        }
+    const size_t nb = g.blocks.size();
+    std::vector<int> type(nb), level(nb), bt(nb), bs(nb), cb(nb), lr(nb), dn(nb);
+    for (size_t i = 0; i < nb; ++i) {
+        const Block& B = g.blocks[i];
+        type[i] = B.type; level[i] = B.level; bt[i] = B.t; bs[i] = B.s;
+        cb[i] = B.childBegin; lr[i] = B.lrIndex; dn[i] = B.denseIndex;
+    }
+    std::vector<const std::vector<int>*> parts = {&lvlBase, &boxOff, &boxSize, &boxChild, &type, &level, &bt,
+                                                  &bs, &cb, &lr, &dn, &g.children, &g.lrBlocks};
+    size_t total = 0;
+    for (auto* v : parts) total += std::max<size_t>(v->size(), 1);
+    std::vector<int> flat;
+    flat.reserve(total);
+    std::vector<size_t> offs;
+    for (auto* v : parts) {
+        offs.push_back(flat.size());
+        flat.insert(flat.end(), v->begin(), v->end());
+        if (v->empty()) flat.push_back(0);
+    }
+    gc.devTree.ensure(flat.size() * sizeof(int));
+    HIPCHECK(copy_sync(gc.devTree.ptr, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
+    const int* d = static_cast<const int*>(gc.devTree.ptr);
+    GeoDev& G = gc.dev;
+    G.dim = g.dim; G.nc = nc; G.numLevels = g.numLevels; G.n = g.n;
+    G.lvlBase = d + offs[0]; G.boxOff = d + offs[1]; G.boxSize = d + offs[2]; G.boxChild = d + offs[3];
+    G.blkType = d + offs[4]; G.blkLevel = d + offs[5]; G.blkT = d + offs[6]; G.blkS = d + offs[7];
+    G.blkChildBegin = d + offs[8]; G.blkLr = d + offs[9]; G.blkDense = d + offs[10]; G.children = d + offs[11];
+    G.lrBlock = d + offs[12];
+    gc.devReady = true;
+    return G;
+}
 
 static std::shared_ptr<GeoCtx> get_geo(int dim, const int* sizes, int L, bool strong)
 {
@@ -1241,49 +1300,55 @@ static std::shared_ptr<OperatorStats> operator_stats(const Geometry& g, const in
     return st;
 }
 
+// HMat2d/HMat3d(Sparse, ...) + ImportSparse (HMat2d.cpp:144-162, 961-1033)
+// on the device: every nonzero walks the block tree to its leaf block
+// (csr_locate); the admissible blocks' nonzero counts (their exact import
+// ranks, ConvertSubmatrix.cpp:57-119) size the layout; then dense values are
+// written in place and each admissible block receives one unit column of U
+// and one value column of V per nonzero, in CSR order (csr_fill).
 static dmhm_hmat_s* create_from_csr(int dim, const int* sizes, int L, int maxRank, bool strong, int prec,
                                     const int* rowptr, const int* colind, const double* vals)
 {
     auto gc = get_geo(dim, sizes, L, strong);
     const Geometry& g = gc->geo;
-    // Import ranks (number of nonzeros inside each admissible block).
-    // Map every nonzero to its leaf-level (row box, col box), then walk up to
-    // find the admissible ancestor block.
-    const int Lf = g.numLevels - 1;
-    std::vector<int> leafBoxOfRow(g.n);
-    for (int b = 0; b < (int)g.boxes[Lf].size(); ++b) {
-        const Box& B = g.boxes[Lf][b];
-        for (int i = 0; i < B.size; ++i) leafBoxOfRow[B.off + i] = b;
+    const int n = g.n;
+    if (rowptr[0] != 0) throw Error(DMHM_EINVAL, "rowOffsets[0] must be 0");
+    for (int i = 0; i < n; ++i)
+        if (rowptr[i + 1] < rowptr[i]) throw Error(DMHM_EINVAL, "rowOffsets must be nondecreasing");
+    const int nnz = rowptr[n];
+    const int nLr = (int)g.lrBlocks.size();
+    const GeoDev& gd = geo_dev(*gc);
+    hipStream_t st = g_lib.stream;
+    // Device scratch of the import: CSR, block per nonzero, counts, staging.
+    DeviceBuffer dRow, dCol, dVal, dBlk, dInt, dStage, dRowOf;
+    dRow.ensure((size_t)(n + 1) * sizeof(int));
+    dCol.ensure((size_t)std::max(nnz, 1) * sizeof(int));
+    dVal.ensure((size_t)std::max(nnz, 1) * sizeof(double));
+    dBlk.ensure((size_t)std::max(nnz, 1) * sizeof(int));
+    dInt.ensure((size_t)(3 * nLr + 4) * sizeof(int));       // counts | bases | fill | bad flag
+    HIPCHECK(copy_sync(dRow.ptr, rowptr, (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (nnz > 0) {
+        HIPCHECK(copy_sync(dCol.ptr, colind, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHECK(copy_sync(dVal.ptr, vals, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice));
     }
-    auto leafOf = [&](int row, int col, int& blockId) {
-        int tb = leafBoxOfRow[row], sb = leafBoxOfRow[col];
-        // ancestors at each level
-        std::vector<int> ta(g.numLevels), sa(g.numLevels);
-        ta[Lf] = tb; sa[Lf] = sb;
-        for (int l = Lf; l > 0; --l) { ta[l - 1] = g.boxes[l][ta[l]].parent; sa[l - 1] = g.boxes[l][sa[l]].parent; }
-        for (int l = 0; l <= Lf; ++l) {
-            const int id = g.find(l, ta[l], sa[l]);
-            if (id >= 0 && g.blocks[id].type != BT_NODE) { blockId = id; return; }
-        }
-        throw Error(DMHM_EINVAL, "internal: no leaf block for a nonzero");
-    };
-    const int nnz = rowptr[g.n];
-    std::vector<int> nzBlock(nnz);
-    std::vector<int> importRank(g.lrBlocks.size(), 0);
-    for (int i = 0; i < g.n; ++i)
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-            if (colind[k] < 0 || colind[k] >= g.n) throw Error(DMHM_EINVAL, "column index out of range");
-            int id; leafOf(i, colind[k], id);
-            nzBlock[k] = id;
-            if (g.blocks[id].type == BT_LOWRANK) importRank[g.blocks[id].lrIndex]++;
-        }
-    bool anyRank = false;
-    std::vector<int> caps(g.lrBlocks.size());
+    int* cnt = static_cast<int*>(dInt.ptr);
+    int* bases = cnt + nLr;
+    int* fill = bases + nLr;
+    int* bad = fill + nLr;
+    launch_fill_int(cnt, 3 * nLr + 4, 0, st);
+    launch_csr_locate(gd, (const int*)dRow.ptr, (const int*)dCol.ptr, n, (int*)dBlk.ptr, cnt, bad, st);
+    std::vector<int> importRank(nLr + 1, 0);
+    int badCount = 0;
+    HIPCHECK(copy_sync(&badCount, bad, sizeof(int), hipMemcpyDeviceToHost));
+    if (badCount > 0) throw Error(DMHM_EINVAL, "column index out of range");
+    if (nLr > 0) HIPCHECK(copy_sync(importRank.data(), cnt, (size_t)nLr * sizeof(int), hipMemcpyDeviceToHost));
+    importRank.resize(nLr);
+    std::vector<int> caps(nLr), base(nLr + 1, 0);
     bool stdCaps = true;
-    for (size_t i = 0; i < caps.size(); ++i) {
+    for (int i = 0; i < nLr; ++i) {
         caps[i] = std::max(std::max(maxRank, 1), importRank[i]);
         stdCaps = stdCaps && caps[i] == std::max(maxRank, 1);
-        anyRank = anyRank || importRank[i] > 0;
+        base[i + 1] = base[i] + importRank[i];
     }
     std::shared_ptr<Layout> lay;
     if (stdCaps) lay = std_layout(*gc, maxRank);
@@ -1292,33 +1357,33 @@ static dmhm_hmat_s* create_from_csr(int dim, const int* sizes, int L, int maxRan
         lay = std::make_shared<Layout>(Layout::make(g, caps, g_lib.nextLayoutId++));
     }
     std::unique_ptr<dmhm_hmat_s> h(alloc_hmat(gc, lay, prec, maxRank));
-    // Host staging of the dense pool (and low-rank pools when ranks > 0).
-    const int64_t hostElems = anyRank ? lay->totalElems() : lay->denseElems;
-    std::vector<double> host(hostElems, 0.0);
-    std::vector<int> fill(g.lrBlocks.size(), 0);
-    for (int i = 0; i < g.n; ++i)
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-            const int id = nzBlock[k];
-            const Block& B = g.blocks[id];
-            const Box& tb = g.tbox(id); const Box& sb = g.sbox(id);
-            const int r = i - tb.off, c = colind[k] - sb.off;
-            if (B.type == BT_DENSE) {
-                host[lay->denseOff[B.denseIndex] + r + (int64_t)c * tb.size] = vals[k];
-            } else {
-                // ConvertSubmatrix (LowRank): U = unit column, V = value column.
-                const int li = B.lrIndex, p = fill[li]++;
-                host[lay->uOff[li] + r + (int64_t)p * tb.size] = 1.0;
-                host[lay->vOff[li] + c + (int64_t)p * sb.size] = vals[k];
-            }
-        }
-    if (prec == DMHM_F64) {
-        HIPCHECK(copy_sync(h->vals, host.data(), hostElems * 8, hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> hf(host.begin(), host.end());
-        HIPCHECK(copy_sync(h->vals, hf.data(), hostElems * 4, hipMemcpyHostToDevice));
+    const int lrTotal = base[nLr];
+    dStage.ensure((size_t)std::max(lrTotal, 1) * sizeof(int));
+    dRowOf.ensure((size_t)std::max(nnz, 1) * sizeof(int));
+    if (nLr > 0) HIPCHECK(copy_sync(bases, base.data(), (size_t)nLr * sizeof(int), hipMemcpyHostToDevice));
+    // Layout offsets for the scatter.
+    DeviceBuffer dLay;
+    const size_t nd = lay->denseOff.size();
+    dLay.ensure((nd + 2 * (size_t)nLr + 1) * sizeof(int64_t));
+    int64_t* lo = static_cast<int64_t*>(dLay.ptr);
+    if (nd) HIPCHECK(copy_sync(lo, lay->denseOff.data(), nd * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (nLr) {
+        HIPCHECK(copy_sync(lo + nd, lay->uOff.data(), (size_t)nLr * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHECK(copy_sync(lo + nd + nLr, lay->vOff.data(), (size_t)nLr * sizeof(int64_t), hipMemcpyHostToDevice));
     }
-    if (!importRank.empty())
-        HIPCHECK(copy_sync(h->ranks, importRank.data(), importRank.size() * sizeof(int), hipMemcpyHostToDevice));
+    LayDev ld;
+    ld.denseOff = lo; ld.uOff = lo + nd; ld.vOff = lo + nd + nLr;
+    if (prec == DMHM_F64)
+        launch_csr_fill<double>(gd, ld, (const int*)dRow.ptr, (const int*)dCol.ptr, (const double*)dVal.ptr, n,
+                                (const int*)dBlk.ptr, bases, fill, cnt, (int*)dStage.ptr, (int*)dRowOf.ptr, nLr,
+                                (double*)h->vals, st);
+    else
+        launch_csr_fill<float>(gd, ld, (const int*)dRow.ptr, (const int*)dCol.ptr, (const double*)dVal.ptr, n,
+                               (const int*)dBlk.ptr, bases, fill, cnt, (int*)dStage.ptr, (int*)dRowOf.ptr, nLr,
+                               (float*)h->vals, st);
+    if (nLr > 0) HIPCHECK(hipMemcpyAsync(h->ranks, cnt, (size_t)nLr * sizeof(int), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (DeviceBuffer* b : {&dRow, &dCol, &dVal, &dBlk, &dInt, &dStage, &dRowOf, &dLay}) b->release();
     h->stats = operator_stats(g, rowptr, colind, vals);
     return h.release();
 }
@@ -1544,6 +1609,22 @@ int dmhm_build_natural_to_hierarchical_map(int dim, const int* sizes, int numLev
         Geometry g(dim, sizes, numLevels, true);
         auto m = g.naturalToHierarchical();
         std::copy(m.begin(), m.end(), map);
+    });
+}
+
+int dmhm_build_natural_to_hierarchical_map_device(int dim, const int* sizes, int numLevels, int* map)
+{
+    return guard([&] {
+        require_ready();
+        if (!sizes || !map) throw Error(DMHM_EINVAL, "null pointer");
+        auto gc = get_geo(dim, sizes, numLevels, true);
+        const GeoDev& gd = geo_dev(*gc);
+        const Geometry& g = gc->geo;
+        DeviceBuffer d;
+        d.ensure((size_t)std::max(g.n, 1) * sizeof(int));
+        launch_natural_map(gd, g.sizes[0], g.sizes[1], g.sizes[2], static_cast<int*>(d.ptr), g_lib.stream);
+        HIPCHECK(copy_sync(map, d.ptr, (size_t)g.n * sizeof(int), hipMemcpyDeviceToHost));
+        d.release();
     });
 }
 

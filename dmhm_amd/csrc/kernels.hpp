@@ -73,6 +73,27 @@ template <typename T> void launch_normalize(T* x, int64_t n, const double* sumsq
                                             double root, hipStream_t s);
 void launch_zero_double(double* x, int n, hipStream_t s);
 
+// Device copy of a block tree for the CSR import (k_build.hip). Boxes are
+// flattened level by level (box (l, id) at lvlBase[l] + id); boxChild holds
+// flattened child indices, nc per box; block arrays follow Geometry::blocks.
+struct GeoDev {
+    int dim = 2, nc = 4, numLevels = 1, n = 0;
+    const int *lvlBase = nullptr, *boxOff = nullptr, *boxSize = nullptr, *boxChild = nullptr;
+    const int *blkType = nullptr, *blkLevel = nullptr, *blkT = nullptr, *blkS = nullptr;
+    const int *blkChildBegin = nullptr, *blkLr = nullptr, *blkDense = nullptr, *children = nullptr;
+    const int* lrBlock = nullptr;
+};
+struct LayDev {
+    const int64_t *denseOff = nullptr, *uOff = nullptr, *vOff = nullptr;
+};
+void launch_csr_locate(const GeoDev& g, const int* rowptr, const int* colind, int n, int* nzBlock, int* lrCount,
+                       int* bad, hipStream_t s);
+template <typename T>
+void launch_csr_fill(const GeoDev& g, const LayDev& lay, const int* rowptr, const int* colind, const double* vals,
+                     int n, const int* nzBlock, const int* lrBase, int* lrFill, const int* lrCount, int* stageK,
+                     int* rowOf, int nLr, T* dst, hipStream_t s);
+void launch_natural_map(const GeoDev& g, int nx, int ny, int nz, int* map, hipStream_t s);
+
 // Rank model (k_mlp.hip): 3-layer perceptron, NF inputs, 64 ReLU units per
 // hidden layer; mlp_param_count(NF) floats of parameters.
 int mlp_param_count(int nf);

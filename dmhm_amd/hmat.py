@@ -31,6 +31,17 @@ def build_natural_to_hierarchical_map(sizes, num_levels):
     return out
 
 
+def build_natural_to_hierarchical_map_device(sizes, num_levels):
+    """The natural -> hierarchical map computed on the GPU."""
+    _lib.init()
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    out = np.zeros(int(np.prod(sizes)), dtype=np.int32)
+    check(_lib.load().dmhm_build_natural_to_hierarchical_map_device(
+        len(sizes), sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), num_levels,
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return out
+
+
 def geometry_block_counts(sizes, num_levels, strongly_admissible, max_levels=12):
     """Dense / low-rank block counts per level of the block tree (host only)."""
     sizes = np.ascontiguousarray(sizes, dtype=np.int32)

@@ -62,6 +62,9 @@ int dmhm_synchronize(void);
  * HMat3d::BuildNaturalToHierarchicalMap (src/HMat3d/HMat3d.cpp:93-111).
  * dim = 2 or 3, sizes[dim]; map[natural] = hierarchical. Host only. */
 int dmhm_build_natural_to_hierarchical_map(int dim, const int* sizes, int numLevels, int* map);
+/* The same map computed on the GPU (one thread per grid point descends the
+ * box tree); map is a host array. Needs dmhm_init. */
+int dmhm_build_natural_to_hierarchical_map_device(int dim, const int* sizes, int numLevels, int* map);
 
 /* Block counts per level of the tree HMat2d/HMat3d builds for a geometry
  * (ImportSparse recursion, src/HMat2d/HMat2d.cpp:961-1033; admissibility
@@ -87,9 +90,10 @@ int dmhm_geometry_product_terms(int dim, const int* sizes, int numLevels, int st
 
 /* HMat2d(const Sparse<Scalar>&, numLevels, maxRank, strong, xSize, ySize)
  * (src/HMat2d/HMat2d.cpp:144-162 + ImportSparse :961-1033) and the HMat3d
- * counterpart. CSR (host arrays) with global hierarchical indices.
- * Admissible blocks are stored exactly (rank = number of nonzeros in the
- * block, ConvertSubmatrix.cpp:57-119). */
+ * counterpart. CSR (host arrays) with global hierarchical indices, imported
+ * on the GPU. Admissible blocks are stored exactly (rank = number of nonzeros
+ * in the block, one unit column of U and one value column of V per nonzero
+ * in CSR order, ConvertSubmatrix.cpp:57-119). */
 int dmhm_hmat_create_from_csr(int dim, const int* sizes, int numLevels, int maxRank, int strong,
                               int precision, const int* rowOffsets, const int* columnIndices,
                               const double* nonzeros, dmhm_hmat* out);

@@ -524,3 +524,39 @@ def test_unpack_rejects_bad_images():
         dmhm_amd.HMat2d.Unpack(img[:-8])
     with pytest.raises(dmhm_amd.DmhmError):
         dmhm_amd.HMat2d.Unpack(b"XXXXXXXX" + img[8:])
+
+
+@pytest.mark.parametrize("sizes,L,strong", [([16, 16], 3, True), ([16, 16], 3, False), ([20, 12], 3, False),
+                                            ([10, 6, 5], 3, False), ([6, 6, 6], 2, True), ([33, 17], 4, False)])
+def test_device_import_matches_oracle_import(sizes, L, strong):
+    """The CSR import on the GPU equals the oracle's ImportSparse restatement
+    exactly (dense leaves and the unit/value low-rank terms expand to the same
+    matrix, bitwise), and every admissible block's rank is its nonzero count
+    (the oracle's import rank)."""
+    op = W.rect_operator(sizes, L)
+    dim = len(sizes)
+    n = int(np.prod(sizes))
+    H = _cls(dim).from_csr(sizes, L, 4, strong, *_args(op))
+    R = orc.HMat.from_csr(dim, sizes, L, 4, strong, *_args(op))
+    np.testing.assert_array_equal(H.to_dense(), R.to_dense())
+    np.testing.assert_array_equal(H.to_dense(), W.csr_to_dense(*_args(op), n))
+    leaves = dmhm_amd.geometry_leaves(sizes, L, strong)
+    rp, ci = op["rowptr"], op["colind"]
+    ranks = H.ranks()
+    for typ, lvl, r0, m, c0, nn, idx in leaves:
+        if typ != 0:
+            continue
+        cnt = 0
+        for i in range(r0, r0 + m):
+            cols = ci[rp[i]:rp[i + 1]]
+            cnt += int(np.count_nonzero((cols >= c0) & (cols < c0 + nn)))
+        assert ranks[idx] == cnt
+
+
+@pytest.mark.parametrize("sizes,L", [([16, 16], 3), ([20, 12], 3), ([7, 9], 2), ([10, 6, 5], 3), ([64, 64], 4),
+                                     ([33, 33, 17], 4)])
+def test_device_natural_map_matches_host(sizes, L):
+    d = dmhm_amd.build_natural_to_hierarchical_map_device(sizes, L)
+    h = dmhm_amd.build_natural_to_hierarchical_map(sizes, L)
+    np.testing.assert_array_equal(d, h)
+    assert sorted(d.tolist()) == list(range(int(np.prod(sizes))))
