@@ -53,7 +53,7 @@ class SchulzOpts(ctypes.Structure):
                 ("theta", ctypes.c_double), ("confidence", ctypes.c_double),
                 ("stopTol", ctypes.c_double), ("mixed", ctypes.c_int), ("switchTol", ctypes.c_double),
                 ("hh", HHOpts), ("stallStop", ctypes.c_int), ("graph", ctypes.c_int), ("bf16", ctypes.c_int),
-                ("bf16Tol", ctypes.c_double)]
+                ("bf16Tol", ctypes.c_double), ("start", ctypes.c_int)]
 
 
 class SchulzReport(ctypes.Structure):
@@ -131,6 +131,17 @@ SIGNATURES = {
     "dmhm_hmat_convert": (ctypes.c_int, [_h, ctypes.c_int, _hp]),
     "dmhm_hmat_estimate_two_norm": (ctypes.c_int, [_h, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _dp]),
     "dmhm_hmat_schulz_invert": (ctypes.c_int, [_h, ctypes.POINTER(SchulzOpts), ctypes.POINTER(SchulzReport)]),
+    "dmhm_hmat_direct_invert": (ctypes.c_int, [_h, ctypes.POINTER(HHOpts)]),
+    "dmhm_hbatch_create": (ctypes.c_int, [ctypes.c_int, _hp, _hp]),
+    "dmhm_hbatch_destroy": (ctypes.c_int, [_h]),
+    "dmhm_hbatch_count": (ctypes.c_int, [_h, _ip]),
+    "dmhm_hbatch_member": (ctypes.c_int, [_h, ctypes.c_int, _hp]),
+    "dmhm_hbatch_multiply_hmat": (ctypes.c_int, [ctypes.c_double, _h, _h, ctypes.POINTER(HHOpts), _hp]),
+    "dmhm_hbatch_estimate_two_norm": (ctypes.c_int, [_h, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _dp]),
+    "dmhm_hbatch_schulz_invert": (ctypes.c_int, [_h, ctypes.POINTER(SchulzOpts), ctypes.POINTER(SchulzReport)]),
+    "dmhm_direct_invert_program_stats": (ctypes.c_int, [_h, ctypes.c_int, _llp, _llp, _dp]),
+    "dmhm_check_direct_invert_program": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                         ctypes.c_int, _llp]),
     "dmhm_hmat_to_dense": (ctypes.c_int, [_h, _dp, ctypes.c_int]),
     "dmhm_hmat_ranks": (ctypes.c_int, [_h, _ip, ctypes.c_int]),
     "dmhm_hmat_matvec_bytes": (ctypes.c_int, [_h, ctypes.c_int, _dp, _dp]),

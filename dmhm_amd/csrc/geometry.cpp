@@ -109,6 +109,7 @@ int Geometry::build(int level, int t, int s)
         denseBlocks.push_back(id);
     }
     blocks[id].leafEnd = (int)leaves.size();
+    blocks[id].blockEnd = (int)blocks.size();
     return id;
 }
 

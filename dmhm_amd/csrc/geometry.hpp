@@ -36,6 +36,7 @@ struct Block {
     int type = BT_NODE;
     int childBegin = -1;          // index into Geometry::children (nc*nc entries)
     int leafBegin = 0, leafEnd = 0; // DFS leaf range (leaves() order)
+    int blockEnd = 0;             // the subtree is blocks [this block, blockEnd) (DFS pre-order)
     int leafIndex = -1;           // position in leaves() for DENSE / LOW_RANK
     int denseIndex = -1;          // index among dense leaves
     int lrIndex = -1;             // index among low-rank blocks

@@ -15,125 +15,27 @@
 #include <string>
 #include <vector>
 
-#include "../../include/dmhm_amd.h"
-#include "geometry.hpp"
-#include "kernels.hpp"
-#include "plan.hpp"
+#include "runtime.hpp"
 
 namespace dmhm {
 
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHECK(x)                                                                       \
-    do {                                                                                  \
-        hipError_t e_ = (x);                                                              \
-        if (e_ != hipSuccess)                                                             \
-            throw Error(e_ == hipErrorOutOfMemory ? DMHM_ENOMEM : DMHM_EHIP,              \
-                        std::string(#x) + ": " + hipGetErrorString(e_));                  \
-    } while (0)
-
 static thread_local std::string g_lastError;
 
-// ---------------------------------------------------------------------------
-// Library state: one device, one stream (default priority), a growing
-// workspace and per-geometry plan caches.
-// ---------------------------------------------------------------------------
-struct DeviceBuffer {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    void ensure(size_t b) {
-        if (b <= bytes) return;
-        if (ptr) HIPCHECK(hipFree(ptr));
-        ptr = nullptr; bytes = 0;
-        HIPCHECK(hipMalloc(&ptr, b));
-        bytes = b;
-    }
-    void release() { if (ptr) (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
-};
-
-struct GeoCtx;
-
-struct EventPair {
-    hipEvent_t a = nullptr, b = nullptr;
-    EventPair() { HIPCHECK(hipEventCreate(&a)); HIPCHECK(hipEventCreate(&b)); }
-    ~EventPair() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
-    float ms() { float t = 0; HIPCHECK(hipEventElapsedTime(&t, a, b)); return t; }
-};
-
-struct Library {
-    bool ready = false;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    DeviceBuffer ws;            // plan workspace
-    DeviceBuffer stageX, stageY;
-    DeviceBuffer featBuf, widthBuf, flagBuf;   // rank-model features, sketch widths, check flags
-    double* dScalars = nullptr; // power-iteration scalars
-    std::vector<std::weak_ptr<GeoCtx>> geos;
-    std::mutex mu;
-    int nextLayoutId = 100;
-    // Tile engine of every TILE phase: -1 autotunes per phase (default);
-    // DMHM_TILE_ENGINE=lds|mfma|mfma_pf|bf16|bf16_pf forces one.
-    int tileEngine = -1;
-    bool bf16 = false;          // BF16 operands allowed (set inside the mixed Schulz loop)
-    long untuned = 0;           // tile launches made while tuning (graph capture waits for none)
-    EventPair* tuneEv = nullptr;
-    // Device iteration counter added to every sketch and start-vector seed
-    // while a replayable loop runs (null otherwise), and a pinned word for the
-    // loop's one readback per iteration.
-    const uint64_t* seedOff = nullptr;
-    uint64_t* dCounter = nullptr;
-    double* hReadback = nullptr;
-};
-static Library g_lib;
+Library g_lib;
 
 // Copies on the library stream and waits for them: the stream is created
 // non-blocking, so a plain hipMemcpy (null stream) would not be ordered after
 // work already queued on it (for example the zero fill of a new matrix).
-static hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
 {
     hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, g_lib.stream);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(g_lib.stream);
 }
 
-static void require_ready()
-{
-    if (!g_lib.ready) throw Error(DMHM_ENODEV, "dmhm_init has not been called");
-}
-
-struct DeviceTables {        // small index tables for per-leaf kernels
-    void* ptr = nullptr;
-    ~DeviceTables() { if (ptr) (void)hipFree(ptr); }
-};
-
-struct OperatorStats;
-struct GeoCtx {
-    Geometry geo;
-    std::shared_ptr<OperatorStats> zeroStats;     // for matrices without an operator's statistics
-    std::map<std::string, std::unique_ptr<Plan>> plans;
-    std::map<int, std::shared_ptr<Layout>> stdLayouts;      // by maxRank
-    // diagonal dense leaves (t == s) for AddConstantToDiagonal, per layout id
-    std::map<int, std::pair<int64_t*, int*>> diagTables;
-    std::map<int, int> diagCount;
-    // Device copy of the block tree (CSR import, index map), built on first use.
-    DeviceBuffer devTree;
-    GeoDev dev;
-    bool devReady = false;
-    GeoCtx(int dim, const int* sizes, int L, bool strong) : geo(dim, sizes, L, strong) {}
-    ~GeoCtx() {
-        for (auto& kv : diagTables) { (void)hipFree(kv.second.first); (void)hipFree(kv.second.second); }
-        devTree.release();
-    }
-};
-
-static hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
-
 // Uploads the block tree of a geometry as the flat int arrays of GeoDev (one
 // allocation), once.
-static const GeoDev& geo_dev(GeoCtx& gc)
+const GeoDev& geo_dev(GeoCtx& gc)
 {
     if (gc.devReady) return gc.dev;
     const Geometry& g = gc.geo;
@@ -183,7 +85,7 @@ static const GeoDev& geo_dev(GeoCtx& gc)
     return G;
 }
 
-static std::shared_ptr<GeoCtx> get_geo(int dim, const int* sizes, int L, bool strong)
+std::shared_ptr<GeoCtx> get_geo(int dim, const int* sizes, int L, bool strong)
 {
     std::lock_guard<std::mutex> lk(g_lib.mu);
     for (auto& w : g_lib.geos) {
@@ -199,7 +101,7 @@ static std::shared_ptr<GeoCtx> get_geo(int dim, const int* sizes, int L, bool st
     return g;
 }
 
-static std::shared_ptr<Layout> std_layout(GeoCtx& gc, int maxRank)
+std::shared_ptr<Layout> std_layout(GeoCtx& gc, int maxRank)
 {
     auto it = gc.stdLayouts.find(maxRank);
     if (it != gc.stdLayouts.end()) return it->second;
@@ -213,41 +115,6 @@ static std::shared_ptr<Layout> std_layout(GeoCtx& gc, int maxRank)
 
 using namespace dmhm;
 
-namespace dmhm {
-// Coefficient statistics of an operator for the rank model: per box of the
-// geometry (every level; box (level, id) at levelBase[level] + id), the mean
-// and standard deviation of log|a_ii| over the box's rows, taken from the
-// diagonal of the imported CSR matrix; and, built on first use, the static
-// features of every admissible block on the device.
-struct OperatorStats {
-    std::vector<int> levelBase;
-    std::vector<float> mean, sdev;
-    float globalMean = 0;
-    DeviceBuffer feat;          // nLowRank x kStaticFeatures floats
-    bool featReady = false;
-    ~OperatorStats() { feat.release(); }
-};
-} // namespace dmhm
-
-struct dmhm_hmat_s {
-    std::shared_ptr<GeoCtx> gc;
-    std::shared_ptr<Layout> lay;
-    std::shared_ptr<OperatorStats> stats;   // of the operator this matrix derives from (or null)
-    int prec = DMHM_F64;
-    int maxRank = 0;
-    void* vals = nullptr;
-    int* ranks = nullptr;
-    // Admissible blocks expected at full capacity rank (products, iterates,
-    // SetToRandom); false for imports and zero matrices, whose rank-0 blocks
-    // the product plans then skip at run time instead of stacking them.
-    bool full = false;
-    size_t esize() const { return prec == DMHM_F64 ? 8 : 4; }
-    ~dmhm_hmat_s() {
-        if (vals) (void)hipFree(vals);
-        if (ranks) (void)hipFree(ranks);
-    }
-    const Geometry& geo() const { return gc->geo; }
-};
 
 // Rank model: the perceptron's parameters and Adam state on the device.
 struct dmhm_rank_model_s {
@@ -267,7 +134,7 @@ constexpr int kRankFeatures = kStaticFeatures + 1;
 struct ProductStats { long long blocks = 0, widthSum = 0, fallbacks = 0, fullWidth = 0; };
 static ProductStats g_lastProduct;
 
-static dmhm_hmat_s* alloc_hmat(std::shared_ptr<GeoCtx> gc, std::shared_ptr<Layout> lay, int prec,
+dmhm_hmat_s* alloc_hmat(std::shared_ptr<GeoCtx> gc, std::shared_ptr<Layout> lay, int prec,
                                int maxRank)
 {
     std::unique_ptr<dmhm_hmat_s> h(new dmhm_hmat_s());
@@ -339,7 +206,7 @@ static void run_tile_phase(Phase& ph, const Bases& b, T alpha, T beta)
 }
 
 template <typename T>
-static void run_phase(Phase& ph, const Bases& b, T alpha, T beta, TruncPolicy pol, uint64_t seed)
+void run_phase(Phase& ph, const Bases& b, T alpha, T beta, TruncPolicy pol, uint64_t seed)
 {
     hipStream_t s = g_lib.stream;
     switch (ph.kind) {
@@ -347,23 +214,33 @@ static void run_phase(Phase& ph, const Bases& b, T alpha, T beta, TruncPolicy po
     case PH_QR: launch_qr_phase<T>(ph, b, s); break;
     case PH_SVD: launch_svd_phase<T>(ph, b, pol, s); break;
     case PH_RNG: launch_rng_phase<T>(ph, b, seed, s, g_lib.seedOff); break;
-    case PH_EYE: launch_eye<T>(reinterpret_cast<T*>(b.val[SLOT_WS]) + ph.zeroOff, (int)ph.zeroCount, s); break;
+    case PH_EYE:
+        if (!ph.eyes.empty())
+            launch_eye_multi<T>(reinterpret_cast<T*>(b.val[SLOT_WS]), static_cast<const int64_t*>(ph.dTasks),
+                                (int)ph.eyes.size(), (int)ph.zeroCount, s);
+        else
+            launch_eye<T>(reinterpret_cast<T*>(b.val[SLOT_WS]) + ph.zeroOff, (int)ph.zeroCount, s);
+        break;
     case PH_MV_ZERO: case PH_MV_PROJ: case PH_MV_ACC: launch_mv_phase<T>(ph, b, alpha, beta, s); break;
     default: break;
     }
 }
 
 template <typename T>
-static void run_plan(Plan& p, const Bases& b, T alpha, T beta, TruncPolicy pol, uint64_t seed)
+void run_plan(Plan& p, const Bases& b, T alpha, T beta, TruncPolicy pol, uint64_t seed)
 {
     for (auto& ph : p.phases) run_phase<T>(ph, b, alpha, beta, pol, seed);
 }
+template void run_phase<double>(Phase&, const Bases&, double, double, TruncPolicy, uint64_t);
+template void run_phase<float>(Phase&, const Bases&, float, float, TruncPolicy, uint64_t);
+template void run_plan<double>(Plan&, const Bases&, double, double, TruncPolicy, uint64_t);
+template void run_plan<float>(Plan&, const Bases&, float, float, TruncPolicy, uint64_t);
 
 // Workspace budget (elements) of one H*H batch: 2^33 (64 GiB in FP64) by
 // default, which keeps the 256^2 rank-16 product in 2-3 batches (103 ms FP64,
 // against 116 ms with 2^31 and 185 ms with 2^29); DMHM_HH_BATCH_ELEMS
 // overrides it (plans built before a change keep theirs).
-static int64_t hh_batch_budget()
+int64_t hh_batch_budget()
 {
     static const int64_t v = [] {
         const char* e = getenv("DMHM_HH_BATCH_ELEMS");
@@ -373,7 +250,7 @@ static int64_t hh_batch_budget()
     return v;
 }
 
-static Plan& get_plan(GeoCtx& gc, const std::string& key, const std::function<Plan*()>& make)
+Plan& get_plan(GeoCtx& gc, const std::string& key, const std::function<Plan*()>& make)
 {
     auto it = gc.plans.find(key);
     if (it != gc.plans.end()) return *it->second;
@@ -384,7 +261,7 @@ static Plan& get_plan(GeoCtx& gc, const std::string& key, const std::function<Pl
     return ref;
 }
 
-static void ensure_ws(const Plan& p, size_t esize)
+void ensure_ws(const Plan& p, size_t esize)
 {
     g_lib.ws.ensure(std::max<size_t>(p.wsElems * esize, 256));
 }
@@ -583,7 +460,6 @@ static dmhm_hmat_s* multiply_hmat(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, 
     return C.release();
 }
 
-static dmhm_hmat_s* copy_hmat(dmhm_hmat_s* s, int prec);
 
 static void multiply_hmat_update(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, double beta, dmhm_hmat_s* C,
                                  const dmhm_hh_opts& o)
@@ -615,7 +491,7 @@ static void multiply_hmat_update(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, d
     C->full = true;
 }
 
-static void update_with(dmhm_hmat_s* A, double alpha, dmhm_hmat_s* B, const dmhm_hh_opts& o)
+void update_with(dmhm_hmat_s* A, double alpha, dmhm_hmat_s* B, const dmhm_hh_opts& o)
 {
     if (A->gc != B->gc) throw Error(DMHM_ENONCONF, "Attempted nonconformal update");
     if (A->prec != B->prec) throw Error(DMHM_EINVAL, "operands must share a precision");
@@ -835,7 +711,7 @@ static dmhm_hmat_s* create_from_lowrank(int dim, const int* sizes, int L, int ma
     return h.release();
 }
 
-static dmhm_hmat_s* copy_hmat(dmhm_hmat_s* s, int prec)
+dmhm_hmat_s* copy_hmat(dmhm_hmat_s* s, int prec)
 {
     std::unique_ptr<dmhm_hmat_s> d(alloc_hmat(s->gc, s->lay, prec, s->maxRank));
     const int64_t n = s->lay->totalElems();
@@ -856,7 +732,7 @@ static void upload(std::vector<T>& v, T** d)
     if (!v.empty()) HIPCHECK(copy_sync(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 
-static dmhm_hmat_s* transpose_hmat(dmhm_hmat_s* s)
+dmhm_hmat_s* transpose_hmat(dmhm_hmat_s* s)
 {
     const Geometry& g = s->geo();
     const Layout& Ls = *s->lay;
@@ -915,7 +791,7 @@ static dmhm_hmat_s* transpose_hmat(dmhm_hmat_s* s)
     return d.release();
 }
 
-static void scale_hmat(dmhm_hmat_s* h, double alpha)
+void scale_hmat(dmhm_hmat_s* h, double alpha)
 {
     const Layout& L = *h->lay;
     hipStream_t st = g_lib.stream;
@@ -931,7 +807,7 @@ static void scale_hmat(dmhm_hmat_s* h, double alpha)
     else launch_scale<float>((float*)h->vals, n, (float)alpha, st);
 }
 
-static void add_diag(dmhm_hmat_s* h, double alpha)
+void add_diag(dmhm_hmat_s* h, double alpha)
 {
     GeoCtx& gc = *h->gc;
     const Geometry& g = gc.geo;
@@ -1029,7 +905,7 @@ static double read_two_norm(double theta)
     return theta * std::exp(g_lib.hReadback[0]);
 }
 
-static double estimate_two_norm(dmhm_hmat_s* h, double theta, double confidence, uint64_t seed)
+double estimate_two_norm(dmhm_hmat_s* h, double theta, double confidence, uint64_t seed)
 {
     if (theta <= 1) throw Error(DMHM_EINVAL, "Theta must be > 1");
     if (confidence <= 0) throw Error(DMHM_EINVAL, "Confidence must be positive");
@@ -1037,11 +913,6 @@ static double estimate_two_norm(dmhm_hmat_s* h, double theta, double confidence,
     return read_two_norm(theta);
 }
 
-// Makes the device iteration counter offset every seed for its lifetime.
-struct SeedOffsetScope {
-    SeedOffsetScope() { g_lib.seedOff = g_lib.dCounter; }
-    ~SeedOffsetScope() { g_lib.seedOff = nullptr; }
-};
 
 // One Schulz step X_out = (2I - X_in A) X_in (Invert-incl.hpp:160-192) as
 // launches only: the counter bump, Z = -X_in A, Z += I, the residual power
@@ -1065,31 +936,6 @@ struct SchulzStep {
         o.seed = seed + 2;
         multiply_into(1.0, Z, Xin, Xout, o);
     }
-};
-
-// A captured step and its executable graph (one per direction of the
-// ping-pong between the two iterate buffers).
-struct StepGraph {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    ~StepGraph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-    }
-    void capture(const SchulzStep& st) {
-        HIPCHECK(hipStreamBeginCapture(g_lib.stream, hipStreamCaptureModeThreadLocal));
-        try {
-            st.launch();
-        } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(g_lib.stream, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
-        }
-        HIPCHECK(hipStreamEndCapture(g_lib.stream, &graph));
-        HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
-    void replay() { HIPCHECK(hipGraphLaunch(exec, g_lib.stream)); }
 };
 
 // SchulzInvert (Invert-incl.hpp:128-196): X_0 = alpha A^T, alpha = 2/est^2,
@@ -1134,10 +980,19 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
     prepare(cur);
     launch_zero_double(reinterpret_cast<double*>(g_lib.dCounter), 1, g_lib.stream);   // counter = 0
     SeedOffsetScope seeds;
-    const double est = estimate_two_norm(ps[cur].A, o.theta, o.confidence, seed);
-    const double alpha = 2.0 / (est * est);
-    std::unique_ptr<dmhm_hmat_s> X0(transpose_hmat(ps[cur].A));
-    scale_hmat(X0.get(), alpha);
+    // X_0: alpha A^T with alpha = 2 / est(||A||_2)^2 (Invert-incl.hpp:150-158),
+    // or the recursive direct inverse (refined by the iteration).
+    double est = 0;
+    std::unique_ptr<dmhm_hmat_s> X0;
+    if (o.start == 1) {
+        X0.reset(copy_hmat(ps[cur].A, ps[cur].A->prec));
+        direct_invert(X0.get(), o.hh);
+    } else {
+        est = estimate_two_norm(ps[cur].A, o.theta, o.confidence, seed);
+        const double alpha = 2.0 / (est * est);
+        X0.reset(transpose_hmat(ps[cur].A));
+        scale_hmat(X0.get(), alpha);
+    }
 
     int it = 0, itLow = 0, itGraph = 0, itBf16 = 0;
     double e = -1;
@@ -1171,7 +1026,7 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
         // mode ran with every phase tuned, later steps replay captured graphs.
         const bool steady = Xc == P.X[0].get() || Xc == P.X[1].get();
         if (o.graph && steady && stepsInPrec >= 2 && P.tuned[bf]) {
-            if (!P.captured[bf][side]) { P.g[bf][side].capture(st); P.captured[bf][side] = true; }
+            if (!P.captured[bf][side]) { P.g[bf][side].capture([&] { st.launch(); }); P.captured[bf][side] = true; }
             P.g[bf][side].replay();
             ++itGraph;
         } else {
@@ -1522,7 +1377,7 @@ static void profile_hh(double alpha, dmhm_hmat_s* A, dmhm_hmat_s* B, const dmhm_
     }
 }
 
-static int guard(const std::function<void()>& fn)
+int guard(const std::function<void()>& fn)
 {
     try {
         fn();
@@ -2120,6 +1975,42 @@ int dmhm_hmat_schulz_invert(dmhm_hmat h, const dmhm_schulz_opts* opts, dmhm_schu
     });
 }
 
+int dmhm_hmat_direct_invert(dmhm_hmat h, const dmhm_hh_opts* opts)
+{
+    return guard([&] {
+        require_ready();
+        if (!h) throw Error(DMHM_EINVAL, "null handle");
+        dmhm_hh_opts o{4, 0, 1e-10, 1, nullptr, 0.0};
+        if (opts) o = *opts;
+        direct_invert(h, o);
+    });
+}
+
+int dmhm_check_direct_invert_program(int dim, const int* sizes, int numLevels, int maxRank, int strong,
+                                     int oversample, long long* stats)
+{
+    return guard([&] {
+        if (!sizes || !stats) throw Error(DMHM_EINVAL, "null pointer");
+        const int over = oversample > 0 ? oversample : 4;
+        if (maxRank < 1 || maxRank + over > 64) throw Error(DMHM_EINVAL, "maxRank + oversample must be <= 64");
+        Geometry g(dim, sizes, numLevels, strong != 0);
+        check_direct_invert_program(g, maxRank, over, stats);
+    });
+}
+
+int dmhm_direct_invert_program_stats(dmhm_hmat h, int oversample, long long* products, long long* plans,
+                                     double* seconds)
+{
+    return guard([&] {
+        require_ready();
+        if (!h || !products || !plans || !seconds) throw Error(DMHM_EINVAL, "null pointer");
+        long p = 0, q = 0;
+        direct_invert_stats(h, oversample, &p, &q, seconds);
+        *products = p;
+        *plans = q;
+    });
+}
+
 int dmhm_hmat_to_dense(dmhm_hmat h, double* out, int ld)
 {
     return guard([&] {
@@ -2375,6 +2266,7 @@ void Plan::upload()
             src = ph.svd.data(); bytes = ph.svd.size() * sizeof(SvdTask);
         }
         if (ph.kind == PH_RNG && !ph.rng.empty()) { src = ph.rng.data(); bytes = ph.rng.size() * sizeof(RngTask); }
+        if (ph.kind == PH_EYE && !ph.eyes.empty()) { src = ph.eyes.data(); bytes = ph.eyes.size() * sizeof(int64_t); }
         if (ph.kind == PH_MV_PROJ && !ph.mvProj.empty()) {
             src = ph.mvProj.data(); bytes = ph.mvProj.size() * sizeof(MvProjItem);
         }

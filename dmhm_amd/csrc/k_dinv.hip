@@ -1,0 +1,149 @@
+// gfx950 kernels of the recursive direct inversion (HMat2d/HMat3d::
+// DirectInvert, src/HMat2d/Invert-incl.hpp:17-125) and of merged plans: the
+// dense leaf inverse (hmat_tools::Invert on a Dense, src/hmat_tools/
+// Invert.cpp:14-42, there LAPACK getrf + getri), copies and fills of index
+// ranges inside one allocation (sub-matrix copies: a subtree of the block
+// tree is one contiguous range of each storage pool), and identities at
+// several workspace offsets.
+#include "kcommon.hpp"
+
+namespace dmhm {
+
+// dst[d + e] = src[s + e] (copy) or dst[d + e] = 0 (fill, src unused) for
+// every range (s, d, len); blockIdx.y selects the range.
+template <typename T, bool FILL>
+__global__ void ranges_kernel(const T* __restrict__ src, T* __restrict__ dst, const CopyRange* __restrict__ rs)
+{
+    const CopyRange r = rs[blockIdx.y];
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < r.len; e += (int64_t)gridDim.x * blockDim.x)
+        dst[r.dst + e] = FILL ? T(0) : src[r.src + e];
+}
+
+template <typename T>
+void launch_copy_ranges(const T* src, T* dst, const CopyRange* ranges, int count, int64_t maxLen, bool fill,
+                        hipStream_t s)
+{
+    if (count <= 0 || maxLen <= 0) return;
+    if (count > 65535) throw std::invalid_argument("too many ranges in one launch");
+    const dim3 grid((unsigned)std::min<int64_t>((maxLen + 255) / 256, 2048), (unsigned)count);
+    if (fill) ranges_kernel<T, true><<<grid, 256, 0, s>>>(src, dst, ranges);
+    else ranges_kernel<T, false><<<grid, 256, 0, s>>>(src, dst, ranges);
+    HIP_LAUNCH_CHECK();
+}
+
+// n x n identities at base + offs[i]; blockIdx.y selects the identity.
+template <typename T>
+__global__ void eye_multi_kernel(T* base, const int64_t* __restrict__ offs, int n)
+{
+    T* x = base + offs[blockIdx.y];
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x)
+        x[e] = (e % n == e / n) ? T(1) : T(0);
+}
+
+template <typename T>
+void launch_eye_multi(T* base, const int64_t* offs, int count, int n, hipStream_t s)
+{
+    if (count <= 0 || n <= 0) return;
+    const dim3 grid((unsigned)std::min((n * n + 255) / 256, 64), (unsigned)count);
+    eye_multi_kernel<T><<<grid, 256, 0, s>>>(base, offs, n);
+    HIP_LAUNCH_CHECK();
+}
+
+// In-place Gauss-Jordan inversion with partial pivoting of one m x m
+// column-major matrix (leading dimension m) by one workgroup of NT threads:
+// at step k the pivot row p (largest |a(i,k)|, i >= k) is swapped into row k,
+// then every entry is updated from the saved column k and the scaled row k;
+// the row swaps are undone as column swaps in reverse order at the end. The
+// matrix is held in LDS when `inLds` (loaded from src, stored to dst), else
+// worked on in dst (copied from src first). A zero pivot sets *singular.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void dense_inverse_kernel(const T* __restrict__ src, T* __restrict__ dst, int m,
+                                                           bool inLds, int* __restrict__ singular)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* col = reinterpret_cast<T*>(smem);              // saved column k (m)
+    T* row = col + m;                                 // scaled row k (m)
+    int* piv = reinterpret_cast<int*>(row + m);       // pivot rows (m)
+    __shared__ int pivot;
+    T* a = inLds ? reinterpret_cast<T*>(piv + ((m + 3) & ~3)) : dst;
+    const int tid = threadIdx.x;
+    const int mm = m * m;
+    for (int e = tid; e < mm; e += NT) a[e] = src[e];
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        if (tid < 64) {
+            // Pivot search by the first wave: largest magnitude, lowest row on ties.
+            double best = -1.0;
+            int bi = k;
+            for (int i = k + tid; i < m; i += 64) {
+                const double v = fabs((double)a[i + (int64_t)k * m]);
+                if (v > best) { best = v; bi = i; }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_xor(best, o, 64);
+                const int oi = __shfl_xor(bi, o, 64);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if (tid == 0) { pivot = bi; piv[k] = bi; }
+        }
+        __syncthreads();
+        const int p = pivot;
+        if (p != k)
+            for (int j = tid; j < m; j += NT) {
+                const T t = a[k + (int64_t)j * m];
+                a[k + (int64_t)j * m] = a[p + (int64_t)j * m];
+                a[p + (int64_t)j * m] = t;
+            }
+        __syncthreads();
+        const T d = a[k + (int64_t)k * m];
+        if (d == T(0) && tid == 0) atomicExch(singular, 1);
+        const T inv = d == T(0) ? T(0) : T(1) / d;
+        for (int i = tid; i < m; i += NT) {
+            col[i] = a[i + (int64_t)k * m];
+            row[i] = a[k + (int64_t)i * m] * inv;
+        }
+        __syncthreads();
+        for (int e = tid; e < mm; e += NT) {
+            const int i = e % m, j = e / m;
+            T v;
+            if (i == k) v = j == k ? inv : row[j];
+            else v = j == k ? -col[i] * inv : a[e] - col[i] * row[j];
+            a[e] = v;
+        }
+        __syncthreads();
+    }
+    for (int k = m - 1; k >= 0; --k) {
+        const int p = piv[k];
+        if (p != k)
+            for (int i = tid; i < m; i += NT) {
+                const T t = a[i + (int64_t)k * m];
+                a[i + (int64_t)k * m] = a[i + (int64_t)p * m];
+                a[i + (int64_t)p * m] = t;
+            }
+        __syncthreads();
+    }
+    if (inLds)
+        for (int e = tid; e < mm; e += NT) dst[e] = a[e];
+}
+
+template <typename T>
+void launch_dense_inverse(const T* src, T* dst, int m, int* singular, hipStream_t s)
+{
+    if (m <= 0) return;
+    const size_t vecs = 2 * (size_t)m * sizeof(T) + (((size_t)m + 3) & ~size_t(3)) * sizeof(int);
+    const size_t full = vecs + (size_t)m * m * sizeof(T);
+    const bool inLds = full <= 64 * 1024;
+    if (vecs > 64 * 1024) throw std::invalid_argument("dense leaf too large to invert");
+    dense_inverse_kernel<T, 512><<<1, 512, inLds ? full : vecs, s>>>(src, dst, m, inLds, singular);
+    HIP_LAUNCH_CHECK();
+}
+
+#define INST(T)                                                                                          \
+    template void launch_copy_ranges<T>(const T*, T*, const CopyRange*, int, int64_t, bool, hipStream_t); \
+    template void launch_eye_multi<T>(T*, const int64_t*, int, int, hipStream_t);                      \
+    template void launch_dense_inverse<T>(const T*, T*, int, int*, hipStream_t);
+INST(double)
+INST(float)
+template void launch_copy_ranges<int>(const int*, int*, const CopyRange*, int, int64_t, bool, hipStream_t);
+
+} // namespace dmhm

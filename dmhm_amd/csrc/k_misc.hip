@@ -266,6 +266,63 @@ template <typename T> void launch_normalize(T* x, int64_t n, const double* sumsq
     HIP_LAUNCH_CHECK();
 }
 
+// Power iteration over a batch of vectors, one per member (segment): the
+// start vectors (element e of member m drawn as element e of a lone vector,
+// so a member's draws do not depend on the batch), and the normalisation with
+// the log of the norm accumulated per member over its first `steps` steps.
+template <typename T>
+__global__ void seg_gaussian_kernel(T* x, const VecSegment* __restrict__ segs, uint64_t seed0, uint64_t stream,
+                                    const uint64_t* seedOff)
+{
+    const VecSegment sg = segs[blockIdx.y];
+    const uint64_t seed = seed_with(seed0, seedOff);
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < sg.n; e += (int64_t)gridDim.x * blockDim.x)
+        x[sg.off + e] = (T)gauss_at(seed, stream, (uint64_t)e);
+}
+template <typename T>
+void launch_seg_gaussian(T* x, const VecSegment* segs, int count, int64_t maxN, uint64_t seed, uint64_t stream,
+                         hipStream_t s, const uint64_t* seedOff)
+{
+    if (count <= 0 || maxN <= 0) return;
+    const dim3 grid((unsigned)std::min<int64_t>((maxN + 255) / 256, 256), (unsigned)count);
+    seg_gaussian_kernel<T><<<grid, 256, 0, s>>>(x, segs, seed, stream, seedOff);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) seg_normalize_kernel(T* x, const VecSegment* __restrict__ segs,
+                                                             double* logAcc, int step)
+{
+    __shared__ double red[16];
+    __shared__ double total;
+    const VecSegment sg = segs[blockIdx.x];
+    T* v = x + sg.off;
+    double s = 0;
+    for (int64_t e = threadIdx.x; e < sg.n; e += 1024) {
+        const double a = (double)v[e];
+        s += a * a;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < 16; ++w) t += red[w];
+        total = t;
+        if (logAcc && step >= 0 && step < sg.steps) logAcc[blockIdx.x] += 0.5 * log(t) / sg.steps;
+    }
+    __syncthreads();
+    const double inv = total > 0 ? 1.0 / sqrt(total) : 0.0;
+    for (int64_t e = threadIdx.x; e < sg.n; e += 1024) v[e] = (T)((double)v[e] * inv);
+}
+template <typename T>
+void launch_seg_normalize(T* x, const VecSegment* segs, int count, double* logAcc, int step, hipStream_t s)
+{
+    if (count <= 0) return;
+    seg_normalize_kernel<T><<<count, 1024, 0, s>>>(x, segs, logAcc, step);
+    HIP_LAUNCH_CHECK();
+}
+
 __global__ void bump_kernel(uint64_t* c) { if (threadIdx.x == 0) *c += 1; }
 void launch_bump(uint64_t* counter, hipStream_t s)
 {
@@ -299,7 +356,10 @@ void launch_zero_double(double* x, int n, hipStream_t s)
                                          const int64_t*, const int*, const int*, const int*, int,   \
                                          hipStream_t);                                              \
     template void launch_sumsq<T>(const T*, int64_t, double*, hipStream_t);                         \
-    template void launch_normalize<T>(T*, int64_t, const double*, double*, double, hipStream_t);
+    template void launch_normalize<T>(T*, int64_t, const double*, double*, double, hipStream_t);          \
+    template void launch_seg_gaussian<T>(T*, const VecSegment*, int, int64_t, uint64_t, uint64_t, hipStream_t, \
+                                         const uint64_t*);                                           \
+    template void launch_seg_normalize<T>(T*, const VecSegment*, int, double*, int, hipStream_t);
 INST(double)
 INST(float)
 template void launch_convert<double, float>(const double*, float*, int64_t, hipStream_t);

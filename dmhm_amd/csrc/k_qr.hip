@@ -4,6 +4,13 @@
 
 namespace dmhm {
 
+// A column whose squared norm (diagonal entry included) is below this is left
+// unreflected (tau = 0): its entries are below ~1e-140, and squares of such
+// entries reach the subnormal range, where a reflector computed from them is
+// no longer orthogonal and would corrupt the later columns. The entries it
+// leaves below the diagonal are that small in absolute terms.
+constexpr double kTinyColumn = 1e-280;
+
 // ---------------------------------------------------------------------------
 // Batched Householder QR (one workgroup per panel, FP64 arithmetic).
 // ---------------------------------------------------------------------------
@@ -56,7 +63,7 @@ __global__ void __launch_bounds__(256) qr_kernel(Bases b, const QrTask* __restri
             const double alpha = (double)A[j + (int64_t)j * m];
             const double xn2 = dots[j];
             double tau = 0.0, scale = 0.0, beta = alpha;
-            if (xn2 > 0.0) {
+            if (xn2 > 0.0 && alpha * alpha + xn2 > kTinyColumn) {
                 beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
                 tau = (beta - alpha) / beta;
                 scale = 1.0 / (alpha - beta);
@@ -180,7 +187,7 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
         for (int q = 0; q < NW; ++q) xn2 += part[buf][q][j];
         const double alpha = rowj[buf][j];
         double t = 0.0, scale = 0.0, beta = alpha;
-        if (xn2 > 0.0) {
+        if (xn2 > 0.0 && alpha * alpha + xn2 > kTinyColumn) {
             beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
             t = (beta - alpha) / beta;
             scale = 1.0 / (alpha - beta);

@@ -67,11 +67,29 @@ template <typename T> void launch_swap_lowrank(const T* src, T* dst, const int64
                                                hipStream_t s);
 void launch_permute_int(const int* src, int* dst, const int* perm, int count, hipStream_t s);
 
+// Direct inversion and merged plans (k_dinv.hip). Ranges are element ranges
+// of one allocation: dst[d + e] = src[s + e] for e < len.
+struct CopyRange { int64_t src, dst, len; };
+template <typename T> void launch_copy_ranges(const T* src, T* dst, const CopyRange* ranges, int count,
+                                              int64_t maxLen, bool fill, hipStream_t s);
+// count n x n identities at base + offs[i] (offs on the device)
+template <typename T> void launch_eye_multi(T* base, const int64_t* offs, int count, int n, hipStream_t s);
+// dst (m x m, ld m) = inverse of src (Gauss-Jordan, partial pivoting); a zero
+// pivot sets *singular (device int).
+template <typename T> void launch_dense_inverse(const T* src, T* dst, int m, int* singular, hipStream_t s);
+
 // Vector utilities for the power iteration (EstimateTwoNorm).
 template <typename T> void launch_sumsq(const T* x, int64_t n, double* out, hipStream_t s);
 template <typename T> void launch_normalize(T* x, int64_t n, const double* sumsq, double* logAcc,
                                             double root, hipStream_t s);
 void launch_zero_double(double* x, int n, hipStream_t s);
+// The same over a batch: member m's vector is x[off, off + n) and takes
+// `steps` power steps (its EstimateTwoNorm count); logAcc[m] accumulates.
+struct VecSegment { int64_t off; int n; int steps; };
+template <typename T> void launch_seg_gaussian(T* x, const VecSegment* segs, int count, int64_t maxN, uint64_t seed,
+                                               uint64_t stream, hipStream_t s, const uint64_t* seedOff = nullptr);
+template <typename T> void launch_seg_normalize(T* x, const VecSegment* segs, int count, double* logAcc, int step,
+                                                hipStream_t s);
 
 // Device copy of a block tree for the CSR import (k_build.hip). Boxes are
 // flattened level by level (box (l, id) at lvlBase[l] + id); boxChild holds

@@ -1,6 +1,7 @@
 #include "plan.hpp"
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <stdexcept>
 
 namespace dmhm {
@@ -237,17 +238,18 @@ Plan* Planner::matvecStream(const Layout& L, int k, int ldx, int ldy)
 // ---------------------------------------------------------------------------
 // H*H product with randomized sum-sketch compression.
 // ---------------------------------------------------------------------------
-ProductTerms enumerateProductTerms(const Geometry& g)
+ProductTerms enumerateProductTerms(const Geometry& g, int rootA, int rootB, int rootC)
 {
     ProductTerms T;
-    const size_t nb = g.blocks.size();
+    T.cBase = rootC;
+    const size_t nb = g.blocks[rootC].blockEnd - rootC;
     T.pushdowns.resize(nb); T.lrTerms.resize(nb); T.nnTerms.resize(nb); T.ddTerms.resize(nb);
     auto addW = [&](int a, int b, int c) {
         T.w.push_back({a, b, c, g.blocks[a].type == BT_LOWRANK});
         return (int)T.w.size() - 1;
     };
     struct Tr { int a, b, c; };
-    std::vector<Tr> stack{{g.root(), g.root(), g.root()}};
+    std::vector<Tr> stack{{rootA, rootB, rootC}};
     while (!stack.empty()) {
         const Tr tr = stack.back();
         stack.pop_back();
@@ -262,24 +264,24 @@ ProductTerms enumerateProductTerms(const Geometry& g)
                         for (int sc = 0; sc < nc; ++sc)
                             stack.push_back({g.child(tr.a, tc, uc), g.child(tr.b, uc, sc), g.child(tr.c, tc, sc)});
             } else {
-                T.pushdowns[tr.c].push_back(addW(tr.a, tr.b, tr.c));
+                T.pushdowns[tr.c - rootC].push_back(addW(tr.a, tr.b, tr.c));
                 T.pushdownTerms++;
             }
         } else if (C.type == BT_LOWRANK) {
             if (A.type == BT_LOWRANK || B.type == BT_LOWRANK) {
-                T.lrTerms[tr.c].push_back(addW(tr.a, tr.b, tr.c));
+                T.lrTerms[tr.c - rootC].push_back(addW(tr.a, tr.b, tr.c));
                 T.lowRankTerms++;
             } else {
-                T.nnTerms[tr.c].push_back({tr.a, tr.b});
+                T.nnTerms[tr.c - rootC].push_back({tr.a, tr.b});
                 if (A.type == BT_DENSE) T.ddToLowRank++;
                 else T.sketchTerms++;
             }
         } else {
             if (A.type == BT_DENSE && B.type == BT_DENSE) {
-                T.ddTerms[tr.c].push_back({tr.a, tr.b});
+                T.ddTerms[tr.c - rootC].push_back({tr.a, tr.b});
                 T.ddToDense++;
             } else {
-                T.lrTerms[tr.c].push_back(addW(tr.a, tr.b, tr.c));
+                T.lrTerms[tr.c - rootC].push_back(addW(tr.a, tr.b, tr.c));
                 T.lowRankTerms++;
             }
         }
@@ -323,12 +325,24 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     };
     auto key2 = [](int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; };
 
+    // The blocks of C written by this plan: the subtree of rootC (all of C
+    // unless sub-matrix roots were set); leaves in DFS order.
+    const Block& rootBlk = g.blocks[rootC_];
+    std::vector<int> cLr, cDense;
+    for (int li = rootBlk.leafBegin; li < rootBlk.leafEnd; ++li) {
+        const int b = g.leaves[li];
+        (g.blocks[b].type == BT_LOWRANK ? cLr : cDense).push_back(b);
+    }
+    if (add && rootC_ != 0) throw std::invalid_argument("the blockwise sum takes whole matrices");
+
     // 1. Enumerate the product recursion; describe every low-rank term.
     ProductTerms terms;
-    if (!add) terms = enumerateProductTerms(g);
+    // Arrays over C's blocks are indexed by b - cBase.
+    const int cBase = rootC_;
+    const size_t nC = rootBlk.blockEnd - cBase;
+    if (!add) terms = enumerateProductTerms(g, rootA_, rootB_, rootC_);
     else {
-        const size_t nb = g.blocks.size();
-        terms.pushdowns.resize(nb); terms.lrTerms.resize(nb); terms.nnTerms.resize(nb); terms.ddTerms.resize(nb);
+        terms.pushdowns.resize(nC); terms.lrTerms.resize(nC); terms.nnTerms.resize(nC); terms.ddTerms.resize(nC);
     }
     const auto& pushdowns = terms.pushdowns;
     const auto& lrTerms = terms.lrTerms;
@@ -362,9 +376,9 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     }
     // beta C_old for every admissible C block (update form); A and alpha B for
     // the blockwise sum (HMat2d::UpdateWith).
-    std::vector<int> oldTerm(g.blocks.size(), -1), addTerm(g.blocks.size(), -1);
+    std::vector<int> oldTerm(nC, -1), addTerm(nC, -1);
     if (add)
-        for (int lb : g.lrBlocks) {
+        for (int lb : cLr) {
             const Block& C = g.blocks[lb];
             for (int which = 0; which < 2; ++which) {
                 const Layout& LX = which ? LB : LA;
@@ -375,12 +389,12 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 w.PU = {slot, LX.uOff[C.lrIndex], g.boxes[C.level][C.t].size};
                 w.PV = {slot, LX.vOff[C.lrIndex], g.boxes[C.level][C.s].size};
                 w.rankIndex = C.lrIndex; w.rankSlot = slot; w.cflag = which ? CF_RUNTIME_ALPHA : 0;
-                if (which == 0) addTerm[lb] = (int)W.size();
+                if (which == 0) addTerm[lb - cBase] = (int)W.size();
                 W.push_back(w);
             }
         }
     if (update)
-        for (int lb : g.lrBlocks) {
+        for (int lb : cLr) {
             const Block& C = g.blocks[lb];
             WFactor w;
             w.level = C.level; w.t = C.t; w.s = C.s;
@@ -388,26 +402,28 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             w.PU = {SLOT_X, LC.uOff[C.lrIndex], g.boxes[C.level][C.t].size};
             w.PV = {SLOT_X, LC.vOff[C.lrIndex], g.boxes[C.level][C.s].size};
             w.rankIndex = C.lrIndex; w.rankSlot = SLOT_X; w.cflag = CF_RUNTIME_BETA;
-            oldTerm[lb] = (int)W.size();
+            oldTerm[lb - cBase] = (int)W.size();
             W.push_back(w);
         }
     // Ancestor pushdowns of every C leaf (ImportLowRank / UpdateWithLowRank
     // distribute a low-rank W over the subtree of a NODE, HMat2d.cpp:811-959).
-    std::vector<std::vector<int>> leafW(g.blocks.size());
-    for (size_t b = 0; b < g.blocks.size(); ++b) {
-        if (pushdowns[b].empty()) continue;
-        const Block& blk = g.blocks[b];
+    std::vector<std::vector<int>> leafWrel(nC);
+    for (size_t r = 0; r < nC; ++r) {
+        if (pushdowns[r].empty()) continue;
+        const Block& blk = g.blocks[cBase + r];
         for (int li = blk.leafBegin; li < blk.leafEnd; ++li) {
-            auto& dst = leafW[g.leaves[li]];
-            dst.insert(dst.end(), pushdowns[b].begin(), pushdowns[b].end());
+            auto& dst = leafWrel[g.leaves[li] - cBase];
+            dst.insert(dst.end(), pushdowns[r].begin(), pushdowns[r].end());
         }
     }
-    for (size_t b = 0; b < g.blocks.size(); ++b) {
-        auto& dst = leafW[b];
-        dst.insert(dst.end(), lrTerms[b].begin(), lrTerms[b].end());
-        if (oldTerm[b] >= 0) dst.push_back(oldTerm[b]);
-        if (addTerm[b] >= 0) { dst.push_back(addTerm[b]); dst.push_back(addTerm[b] + 1); }
+    for (size_t r = 0; r < nC; ++r) {
+        auto& dst = leafWrel[r];
+        dst.insert(dst.end(), lrTerms[r].begin(), lrTerms[r].end());
+        if (oldTerm[r] >= 0) dst.push_back(oldTerm[r]);
+        if (addTerm[r] >= 0) { dst.push_back(addTerm[r]); dst.push_back(addTerm[r] + 1); }
     }
+    auto leafW = [&](int b) -> const std::vector<int>& { return leafWrel[b - cBase]; };
+    auto nnOf = [&](int b) -> const std::vector<std::pair<int, int>>& { return nnTerms[b - cBase]; };
 
     // 2. Batches. Every term lands in one C block. The C blocks are grouped
     //    into units: the subtrees rooted at level `unitLevel`, and the leaves
@@ -418,43 +434,44 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     //    unitLevel starts at 1 and deepens until every unit's estimated
     //    workspace fits the budget.
     const double budget = (double)wsBudget_;
-    std::vector<int> unit(g.blocks.size(), -1);
+    std::vector<int> unitRel(nC, -1);
+    auto unit = [&](int b) -> int& { return unitRel[b - cBase]; };
     int nUnits = 0;
     std::vector<double> est;
-    for (int unitLevel = 1;; ++unitLevel) {
-        std::fill(unit.begin(), unit.end(), -1);
+    for (int unitLevel = rootBlk.level + 1;; ++unitLevel) {
+        std::fill(unitRel.begin(), unitRel.end(), -1);
         nUnits = 0;
-        // DFS from the root: a block at unitLevel (or a leaf above it) opens a unit.
-        std::vector<std::pair<int, int>> stack{{g.root(), -1}};
+        // DFS from C's root: a block at unitLevel (or a leaf above it) opens a unit.
+        std::vector<std::pair<int, int>> stack{{rootC_, -1}};
         while (!stack.empty()) {
             const auto [b, u] = stack.back();
             stack.pop_back();
             const Block& B = g.blocks[b];
             int ub = u;
             if (ub < 0 && (B.level >= unitLevel || B.type != BT_NODE)) ub = nUnits++;
-            unit[b] = ub;
+            unit(b) = ub;
             if (B.type == BT_NODE)
                 for (int c = g.nc() * g.nc() - 1; c >= 0; --c) stack.push_back({g.children[B.childBegin + c], ub});
         }
         // Workspace estimate per unit (elements), used only to group units.
         est.assign(nUnits, 0.0);
-        for (int lbk : g.lrBlocks) {
+        for (int lbk : cLr) {
             const Block& C = g.blocks[lbk];
             const int m = g.boxes[C.level][C.t].size, n = g.boxes[C.level][C.s].size;
             double e = (double)(m + n) * kS + 3.0 * kS * kS;
             if (C.level == g.numLevels - 1) e += (double)m * n;
-            e += 2.0 * kS * 64 * leafW[lbk].size();
-            for (const auto& tm : nnTerms[lbk]) {
+            e += 2.0 * kS * 64 * leafW(lbk).size();
+            for (const auto& tm : nnOf(lbk)) {
                 const Block& A = g.blocks[tm.first];
                 e += (double)g.boxes[A.level][A.s].size * kS + 2.0 * 64 * kS * (A.leafEnd - A.leafBegin);
             }
-            est[unit[lbk]] += e;
+            est[unit(lbk)] += e;
         }
         for (const auto& wp : wPending) {
-            if (unit[wp.bC] < 0) continue;
+            if (unit(wp.bC) < 0) continue;
             const WFactor& w = W[wp.wIndex];
             const Block& Bl = g.blocks[wp.left ? wp.bB : wp.bA];
-            est[unit[wp.bC]] += (double)(wp.left ? w.PV.ld : w.PU.ld) * w.cap +
+            est[unit(wp.bC)] += (double)(wp.left ? w.PV.ld : w.PU.ld) * w.cap +
                                 64.0 * kS * (Bl.leafEnd - Bl.leafBegin);
         }
         const double worst = est.empty() ? 0.0 : *std::max_element(est.begin(), est.end());
@@ -472,7 +489,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             acc += est[i];
         }
     }
-    auto inBatch = [&](const std::vector<char>& grp, int b) { return unit[b] >= 0 && grp[unit[b]]; };
+    auto inBatch = [&](const std::vector<char>& grp, int b) { return unit(b) >= 0 && grp[unit(b)]; };
 
     plan->phases.reserve(4 + 18 * groups.size());
     // Exact factor of a low-rank term: the side that is not a stored factor
@@ -511,7 +528,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     }
     {
         Phase& rngPh = newPhase(PH_RNG);
-        for (int lbk : g.lrBlocks) {
+        for (int lbk : cLr) {
             const Block& C = g.blocks[lbk];
             if (omegaOf.count(key2(C.level, C.s))) continue;
             const int n = g.boxes[C.level][C.s].size;
@@ -526,11 +543,11 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         Phase& a0 = newPhase(PH_TILE);
         PhaseBuilder proj(p0), acc(a0);
         for (const auto& wp : wPending)
-            if (unit[wp.bC] < 0) buildW(proj, acc, wp);
-        for (int lbk : g.lrBlocks) {
+            if (unit(wp.bC) < 0) buildW(proj, acc, wp);
+        for (int lbk : cLr) {
             const Block& C = g.blocks[lbk];
             if (C.level == g.numLevels - 1) continue;
-            for (const auto& tm : nnTerms[lbk]) {
+            for (const auto& tm : nnOf(lbk)) {
                 if (t1Of.count((uint64_t)tm.second)) continue;
                 const Block& B = g.blocks[tm.second];
                 const Box& ub = g.boxes[B.level][B.t];
@@ -593,7 +610,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             }
             plan->flops += 2.0 * tb.size * usz * sb.size;
         }
-        for (int wi : leafW[b]) {
+        for (int wi : leafW(b)) {
             const WFactor& w = W[wi];
             const Operand pu = slicePU(w, tb), pv = slicePV(w, sb);
             for (int a = tb.atomBegin; a < tb.atomEnd; ++a) {
@@ -664,7 +681,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     for (const auto& inGroup : groups) {
         ws_ = globalEnd;
         std::vector<Sk> sk;
-        for (int lbk : g.lrBlocks) {
+        for (int lbk : cLr) {
             if (!inBatch(inGroup, lbk)) continue;
             const Block& C = g.blocks[lbk];
             const Box& tb = g.boxes[C.level][C.t];
@@ -759,15 +776,15 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
             Phase& ad = newPhase(PH_TILE);
             PhaseBuilder acc(ad);
             if (!add)
-                for (int bd : g.denseBlocks) {
+                for (int bd : cDense) {
                     if (!inBatch(inGroup, bd)) continue;
                     const Block& C = g.blocks[bd];
                     // Dense C leaves: C = alpha (sum of terms) (+ beta C_old, read in place).
                     denseAccumulate(acc, bd, {SLOT_C, LC.denseOff[C.denseIndex], g.boxes[C.level][C.t].size},
-                                    ddTerms[bd], CF_RUNTIME_ALPHA, update ? TF_RUNTIME_BETA : 0);
+                                    ddTerms[bd - cBase], CF_RUNTIME_ALPHA, update ? TF_RUNTIME_BETA : 0);
                 }
             for (const auto& s : sk)
-                if (s.leaf) denseAccumulate(acc, s.b, {SLOT_WS, s.D, s.m}, nnTerms[s.b], abFlag, 0);
+                if (s.leaf) denseAccumulate(acc, s.b, {SLOT_WS, s.D, s.m}, nnOf(s.b), abFlag, 0);
             acc.finalize();
         }
         // Y = T Omega for admissible C blocks.
@@ -791,7 +808,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                     plan->flops += 2.0 * s.m * s.n * kS;
                     continue;
                 }
-                for (int wi : leafW[s.b]) {
+                for (int wi : leafW(s.b)) {
                     const WFactor& w = W[wi];
                     const int64_t P = wsAlloc((int64_t)w.cap * kS);
                     projTile(proj, slicePV(w, sb), sb, om, w, P, s.wr);
@@ -804,7 +821,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                     }
                     plan->flops += 2.0 * tb.size * w.cap * kS;
                 }
-                for (const auto& tm : nnTerms[s.b]) {
+                for (const auto& tm : nnOf(s.b)) {
                     const Block& B = g.blocks[tm.second];
                     const Operand in{SLOT_WS, t1Of[(uint64_t)tm.second], g.boxes[B.level][B.t].size};
                     subh(proj, acc, SLOT_A, LA, tm.first, false, in, B.level, B.t, pt, kS, s.wr, SLOT_WIDTH, 1.0f,
@@ -853,7 +870,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                     plan->flops += 2.0 * s.m * s.n * kS;
                     continue;
                 }
-                for (int wi : leafW[s.b]) {
+                for (int wi : leafW(s.b)) {
                     const WFactor& w = W[wi];
                     const int64_t P = wsAlloc((int64_t)w.cap * kS);
                     projTile(proj3, slicePU(w, tb), tb, Q, w, P, s.wr);
@@ -866,7 +883,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                     }
                     plan->flops += 2.0 * sb.size * w.cap * kS;
                 }
-                for (const auto& tm : nnTerms[s.b]) {
+                for (const auto& tm : nnOf(s.b)) {
                     const Block& A = g.blocks[tm.first];
                     const Box& ub = g.boxes[A.level][A.s];
                     const int64_t T2 = wsAlloc((int64_t)ub.size * kS);
@@ -948,6 +965,202 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
         plan->contribCount += ph.contribs.size();
     }
     return plan;
+}
+
+// ---------------------------------------------------------------------------
+// Merged plans
+// ---------------------------------------------------------------------------
+Plan* merge_plans(const std::vector<PlanPart>& parts)
+{
+    std::unique_ptr<Plan> out(new Plan());
+    const size_t np = parts.size();
+    std::vector<int64_t> wsBase(np);
+    int64_t ws = 0;
+    for (size_t i = 0; i < np; ++i) {
+        wsBase[i] = ws;
+        ws += (parts[i].plan->wsElems + 63) & ~int64_t(63);
+        out->flops += parts[i].plan->flops;
+        out->batches = std::max(out->batches, parts[i].plan->batches);
+    }
+    out->wsElems = ws;
+    // Shifts of part i: the workspace slot goes to the part's region.
+    auto vsh = [&](size_t i, int slot) { return slot == SLOT_WS ? wsBase[i] : parts[i].val[slot]; };
+    auto rsh = [&](size_t i, int slot, int idx) { return idx < 0 ? idx : idx + (int)parts[i].rank[slot]; };
+
+    std::vector<size_t> pos(np, 0);
+    for (;;) {
+        // The next merged phase takes the kind of the part with the most
+        // phases left, and every part whose next phase has that kind.
+        int lead = -1;
+        size_t most = 0;
+        for (size_t i = 0; i < np; ++i) {
+            const size_t left = parts[i].plan->phases.size() - pos[i];
+            if (left > most) { most = left; lead = (int)i; }
+        }
+        if (lead < 0) break;
+        const Phase& lp = parts[lead].plan->phases[pos[lead]];
+        out->phases.emplace_back();
+        Phase& m = out->phases.back();
+        m.kind = lp.kind;
+        m.mvK = lp.mvK; m.ldx = lp.ldx; m.ldy = lp.ldy;
+        m.zeroCount = lp.zeroCount;
+        m.role = lp.role;
+        bool first = true;
+        for (size_t i = 0; i < np; ++i) {
+            if (pos[i] >= parts[i].plan->phases.size()) continue;
+            const Phase& ph = parts[i].plan->phases[pos[i]];
+            if (ph.kind != m.kind) continue;
+            ++pos[i];
+            m.flops += ph.flops;
+            m.qrK = std::max(m.qrK, ph.qrK);
+            switch (ph.kind) {
+            case PH_TILE: {
+                const int cbase = (int)m.contribs.size();
+                for (Tile t : ph.tiles) {
+                    t.off += vsh(i, t.slot);
+                    t.mrank = rsh(i, t.mslot, t.mrank);
+                    t.krank = rsh(i, t.kslot, t.krank);
+                    t.cb += cbase; t.ce += cbase;
+                    m.tiles.push_back(t);
+                    m.maxKTile = std::max(m.maxKTile, t.k);
+                    m.maxMTile = std::max(m.maxMTile, t.m);
+                }
+                for (Contrib c : ph.contribs) {
+                    c.loff += vsh(i, c.lslot);
+                    c.roff += vsh(i, c.rslot);
+                    c.qrank = rsh(i, c.qslot, c.qrank);
+                    m.contribs.push_back(c);
+                }
+                break;
+            }
+            case PH_QR:
+                for (QrTask t : ph.qr) {
+                    t.off += vsh(i, t.slot);
+                    if (t.roff >= 0) t.roff += wsBase[i];
+                    t.krank = rsh(i, t.kslot, t.krank);
+                    m.qr.push_back(t);
+                }
+                break;
+            case PH_SVD:
+                for (SvdTask t : ph.svd) {
+                    t.roff += wsBase[i]; t.g1off += wsBase[i]; t.g2off += wsBase[i];
+                    t.rankIndex = rsh(i, SLOT_C, t.rankIndex);
+                    t.krank = rsh(i, SLOT_WIDTH, t.krank);
+                    m.svd.push_back(t);
+                }
+                break;
+            case PH_RNG:
+                for (RngTask t : ph.rng) {
+                    t.off += wsBase[i];
+                    m.rng.push_back(t);
+                }
+                break;
+            case PH_EYE:
+                if (ph.zeroCount != m.zeroCount) throw std::logic_error("merged identities of different orders");
+                if (ph.eyes.empty()) m.eyes.push_back(ph.zeroOff + wsBase[i]);
+                for (int64_t e : ph.eyes) m.eyes.push_back(e + wsBase[i]);
+                break;
+            case PH_MV_ZERO:
+                // A streaming product zeroes the front of its workspace (its
+                // projections); the merged phase zeroes the whole workspace.
+                if (ph.zeroOff != 0 || ph.zeroCount > parts[i].plan->wsElems)
+                    throw std::logic_error("merged zero range is not inside the part's workspace");
+                m.zeroOff = 0;
+                m.zeroCount = -1;
+                break;
+            case PH_MV_PROJ:
+                if (!first && (ph.mvK != m.mvK || (ph.mvK > 1 && (ph.ldx != m.ldx || ph.ldy != m.ldy))))
+                    throw std::logic_error("merged streaming products of different shapes");
+                for (MvProjItem it : ph.mvProj) {
+                    it.voff += vsh(i, SLOT_A);
+                    it.xrow += vsh(i, SLOT_X);
+                    it.poff += wsBase[i];
+                    it.rankIdx = rsh(i, SLOT_A, it.rankIdx);
+                    m.mvProj.push_back(it);
+                }
+                break;
+            case PH_MV_ACC: {
+                if (!first && (ph.mvK != m.mvK || (ph.mvK > 1 && (ph.ldx != m.ldx || ph.ldy != m.ldy))))
+                    throw std::logic_error("merged streaming products of different shapes");
+                const int cbase = (int)m.mvContribs.size();
+                for (MvAtom a : ph.mvAtoms) {
+                    a.yrow += vsh(i, SLOT_Y);
+                    a.cb += cbase; a.ce += cbase;
+                    m.mvAtoms.push_back(a);
+                }
+                for (MvContrib c : ph.mvContribs) {
+                    c.loff += vsh(i, SLOT_A);
+                    c.roff += c.kind == 0 ? vsh(i, SLOT_X) : wsBase[i];
+                    c.rankIdx = rsh(i, SLOT_A, c.rankIdx);
+                    m.mvContribs.push_back(c);
+                }
+                break;
+            }
+            default:
+                throw std::logic_error("phase kind cannot be merged");
+            }
+            first = false;
+        }
+    }
+    for (auto& ph : out->phases) {
+        if (ph.kind == PH_MV_ZERO) ph.zeroCount = out->wsElems;
+        out->tileCount += ph.tiles.size();
+        out->contribCount += ph.contribs.size();
+    }
+    return out.release();
+}
+
+std::string check_plan(const Plan& p, const int64_t* slotElems, const int64_t* rankElems)
+{
+    auto size = [&](int slot) { return slot == SLOT_WS ? p.wsElems : slotElems[slot]; };
+    auto where = [&](size_t ph, const char* what) {
+        return "phase " + std::to_string(ph) + " " + what;
+    };
+    // Extent of an r x c column-major region at off with leading dimension ld.
+    auto inside = [&](int slot, int64_t off, int64_t ld, int64_t r, int64_t c) {
+        if (r <= 0 || c <= 0) return true;
+        if (slot < 0 || slot >= NUM_SLOTS || off < 0 || ld < r) return false;
+        return off + (c - 1) * ld + r <= size(slot);
+    };
+    auto rankOk = [&](int slot, int idx) { return idx < 0 || (slot < NUM_SLOTS && idx < rankElems[slot]); };
+    for (size_t i = 0; i < p.phases.size(); ++i) {
+        const Phase& ph = p.phases[i];
+        for (const Tile& t : ph.tiles) {
+            if (!inside(t.slot, t.off, t.ld, t.m, t.k)) return where(i, "tile output out of bounds");
+            if (!rankOk(t.mslot, t.mrank) || !rankOk(t.kslot, t.krank)) return where(i, "tile rank index");
+            if (t.cb < 0 || t.ce > (int)ph.contribs.size() || t.cb > t.ce) return where(i, "tile contribution range");
+            for (int q = t.cb; q < t.ce; ++q) {
+                const Contrib& c = ph.contribs[q];
+                const bool tl = c.flags & CF_TRANS_L, tr = c.flags & CF_TRANS_R;
+                const bool okL = tl ? inside(c.lslot, c.loff, c.ldl, c.q, t.m) : inside(c.lslot, c.loff, c.ldl, t.m, c.q);
+                const bool okR = tr ? inside(c.rslot, c.roff, c.ldr, t.k, c.q) : inside(c.rslot, c.roff, c.ldr, c.q, t.k);
+                if (!okL) return where(i, "left operand out of bounds");
+                if (!okR) return where(i, "right operand out of bounds");
+                if (!rankOk(c.qslot, c.qrank)) return where(i, "contribution rank index");
+            }
+        }
+        for (const QrTask& t : ph.qr) {
+            if (!inside(t.slot, t.off, t.ld, t.m, t.k)) return where(i, "QR panel out of bounds");
+            if (t.roff >= 0 && !inside(SLOT_WS, t.roff, t.rld, t.k, t.k)) return where(i, "QR R factor out of bounds");
+            if (!rankOk(t.kslot, t.krank)) return where(i, "QR rank index");
+        }
+        for (const SvdTask& t : ph.svd) {
+            const int ld = t.ld > 0 ? t.ld : t.k;
+            if (!inside(SLOT_WS, t.roff, ld, t.k, t.k) || !inside(SLOT_WS, t.g1off, ld, t.k, t.cap) ||
+                !inside(SLOT_WS, t.g2off, ld, t.k, t.cap))
+                return where(i, "SVD factor out of bounds");
+            if (!rankOk(SLOT_C, t.rankIndex) || t.rankIndex < 0) return where(i, "SVD rank index");
+        }
+        for (const RngTask& t : ph.rng)
+            if (!inside(SLOT_WS, t.off, t.m, t.m, t.k)) return where(i, "sketch panel out of bounds");
+        if (ph.kind == PH_EYE) {
+            std::vector<int64_t> eyes = ph.eyes;
+            if (eyes.empty()) eyes.push_back(ph.zeroOff);
+            for (int64_t e : eyes)
+                if (!inside(SLOT_WS, e, ph.zeroCount, ph.zeroCount, ph.zeroCount)) return where(i, "identity out of bounds");
+        }
+    }
+    return std::string();
 }
 
 } // namespace dmhm

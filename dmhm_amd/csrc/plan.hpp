@@ -19,6 +19,7 @@
 // and whole iterations can be captured in a HIP graph.
 #pragma once
 #include <cstdint>
+#include <string>
 #include <vector>
 #include "geometry.hpp"
 
@@ -120,6 +121,7 @@ struct Phase {
     std::vector<MvAtom> mvAtoms;
     std::vector<MvContrib> mvContribs;
     int64_t zeroOff = 0, zeroCount = 0;   // PH_MV_ZERO range in the workspace; PH_EYE: offset, order
+    std::vector<int64_t> eyes;            // PH_EYE of a merged plan: every identity's workspace offset
     int mvK = 0, ldx = 0, ldy = 0;
     void* dAux = nullptr;
     int qrK = 0;            // panel width of the QR phase
@@ -183,12 +185,38 @@ struct Operand {            // a panel or matrix block region
 struct ProductTerms {
     struct W { int bA, bB, bC; bool leftLowRank; };
     std::vector<W> w;                              // all low-rank terms
+    // Per C block b of the subtree of rootC, at index b - cBase:
+    int cBase = 0;
     std::vector<std::vector<int>> pushdowns;       // per C NODE block: w ids
     std::vector<std::vector<int>> lrTerms;         // per C leaf block: w ids
     std::vector<std::vector<std::pair<int, int>>> nnTerms, ddTerms;  // per C block (bA, bB)
     long sketchTerms = 0, ddToLowRank = 0, ddToDense = 0, pushdownTerms = 0, lowRankTerms = 0;
 };
-ProductTerms enumerateProductTerms(const Geometry& g);
+// The recursion starts at the block triple (rootA, rootB, rootC) (the whole
+// matrices: the root block three times).
+ProductTerms enumerateProductTerms(const Geometry& g, int rootA = 0, int rootB = 0, int rootC = 0);
+
+// One part of a merged plan: a plan and, per slot, the element offset added to
+// every operand offset in that slot and the index offset added to every rank
+// index read from that slot's rank array (SLOT_WS is placed by the merge).
+struct PlanPart {
+    const Plan* plan = nullptr;
+    int64_t val[NUM_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t rank[NUM_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+// Merges plans over disjoint data into one plan with one launch per merged
+// phase: each part keeps its phase order, phases of the same kind from
+// different parts share a launch (tasks concatenated and shifted), and every
+// part gets its own workspace region. The parts must not be uploaded (their
+// host task arrays are read) and must not write what another part reads.
+Plan* merge_plans(const std::vector<PlanPart>& parts);
+
+// Host-side bounds check of a plan that has not been uploaded: every operand,
+// output, QR panel, SVD factor, sketch panel and identity of every task lies
+// inside its slot (slotElems[slot] elements; SLOT_WS: the plan's workspace)
+// and every rank index inside rankElems[slot]. Returns an empty string or the
+// first violation.
+std::string check_plan(const Plan& p, const int64_t* slotElems, const int64_t* rankElems);
 
 // Plan builders.
 class Planner {
@@ -229,6 +257,10 @@ public:
     // Whether the low-rank factors of A / B are expected at full rank, so that
     // the exact-factor terms they enter are stacked (see hhProduct).
     void setStacking(bool a, bool b) { stackA_ = a; stackB_ = b; }
+    // Sub-matrix products: C(rootC) = A(rootA) B(rootB) for blocks (t,u), (u,s),
+    // (t,s) of one level (HMat2d::Multiply on child matrices,
+    // MultiplyHMat-incl.hpp:12-380); only the blocks of C's subtree are written.
+    void setRoots(int a, int b, int c) { rootA_ = a; rootB_ = b; rootC_ = c; }
     // Width-indirect sketches: every admissible C block sketches with
     // min(maxRank + oversample, widths[block]) columns, widths read at run time
     // from the SLOT_WIDTH array (indexed like C's ranks); the SVD then flags
@@ -259,6 +291,7 @@ private:
     int tsqrRows_ = 0;
     bool stackA_ = true, stackB_ = true;
     bool widthIndirect_ = false;
+    int rootA_ = 0, rootB_ = 0, rootC_ = 0;
 };
 
 } // namespace dmhm

@@ -342,21 +342,44 @@ class _HMatBase:
     def SchulzInvert(self, numIterations=-1, theta=1.5, confidence=6.0, stopTol=1e-4, maxIterations=30,
                      mixed=False, switchTol=1e-3, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1,
                      raise_on_noconv=False, stall_stop=False, graph=True, bf16=False, bf16Tol=0.5,
-                     rank_model=None, rank_margin=1.0):
+                     rank_model=None, rank_margin=1.0, start="transpose"):
         """A := inv(A) by Schulz iteration (Invert-incl.hpp:128-196). Returns the
         report. graph=True replays captured HIP graphs for the steady steps;
         mixed=True runs FP32 steps until the estimate is below switchTol, with
         BF16 tile operands while it is at least bf16Tol when bf16=True;
-        rank_model (with tol_mode=1) sizes every product's sketches."""
+        rank_model (with tol_mode=1) sizes every product's sketches.
+        start="transpose" is the reference's X_0 = alpha A^T; start="direct"
+        takes X_0 from DirectInvert, which the iteration then refines."""
+        if start not in ("transpose", "direct"):
+            raise ValueError("start must be 'transpose' or 'direct'")
         o = _lib.SchulzOpts(numIterations, maxIterations, theta, confidence, stopTol, 1 if mixed else 0,
                             switchTol, _hh_opts(oversample, tol_mode, rel_tol, seed, rank_model, rank_margin),
-                            1 if stall_stop else 0, 1 if graph else 0, 1 if bf16 else 0, bf16Tol)
+                            1 if stall_stop else 0, 1 if graph else 0, 1 if bf16 else 0, bf16Tol,
+                            1 if start == "direct" else 0)
         rep = _lib.SchulzReport()
         code = _lib.load().dmhm_hmat_schulz_invert(self._h, ctypes.byref(o), ctypes.byref(rep))
         if code == 4 and not raise_on_noconv:
             return rep
         check(code)
         return rep
+
+    def DirectInvert(self, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1):
+        """A := inv(A) by recursive block Gauss-Jordan elimination
+        (HMat2d::DirectInvert, src/HMat2d/Invert-incl.hpp:17-125); every
+        sub-matrix product is truncated to maxRank (or by rel_tol). Raises
+        DmhmError (DMHM_ESINGULAR) on a zero pivot."""
+        o = _hh_opts(oversample, tol_mode, rel_tol, seed)
+        check(_lib.load().dmhm_hmat_direct_invert(self._h, ctypes.byref(o)))
+        return self
+
+    def direct_invert_program(self, oversample=4):
+        """(sub-matrix products, merged plans, host build seconds) of the
+        cached direct-inversion program of this geometry and rank."""
+        p, q = ctypes.c_longlong(), ctypes.c_longlong()
+        s = ctypes.c_double()
+        check(_lib.load().dmhm_direct_invert_program_stats(self._h, oversample, ctypes.byref(p), ctypes.byref(q),
+                                                            ctypes.byref(s)))
+        return p.value, q.value, s.value
 
     def profile_matvec(self, k=1, reps=20):
         """HIP-event timing of H*X on the library stream: (ms, bytes) arrays."""
@@ -542,3 +565,79 @@ def Compress(max_rank, D):
 def EstimateTwoNorm(A, theta=1.5, confidence=6.0, seed=1):
     """hmat_tools::EstimateTwoNorm (include/dmhm/hmat_tools.hpp:1504-1544)."""
     return A.EstimateTwoNorm(theta, confidence, seed)
+
+
+class HBatch:
+    """Independent H-matrices worked on together, one launch per phase for
+    all members (dmhm_hbatch_*). No reference counterpart: the reference
+    inverts such operators one by one (src/HMat3d/Invert-incl.hpp:127-188)."""
+
+    def __init__(self, handle, cls_of):
+        self._h = ctypes.c_void_p(handle) if not isinstance(handle, ctypes.c_void_p) else handle
+        self._cls = list(cls_of)
+
+    @classmethod
+    def create(cls, mats):
+        _lib.init()
+        arr = (ctypes.c_void_p * len(mats))(*[m._h.value for m in mats])
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hbatch_create(len(mats), arr, ctypes.byref(out)))
+        return cls(out, [type(m) for m in mats])
+
+    def __del__(self):
+        try:
+            if self._h and self._h.value:
+                _lib.load().dmhm_hbatch_destroy(self._h)
+                self._h = ctypes.c_void_p()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self._cls)
+
+    def member(self, i):
+        """A new handle holding a copy of member i."""
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hbatch_member(self._h, i, ctypes.byref(out)))
+        return self._cls[i](out)
+
+    def Multiply(self, alpha, B, oversample=4, tol_mode=0, rel_tol=1e-10, seed=1):
+        """C_i = alpha A_i B_i for every member (HMat2d::Multiply per member)."""
+        o = _hh_opts(oversample, tol_mode, rel_tol, seed)
+        out = ctypes.c_void_p()
+        check(_lib.load().dmhm_hbatch_multiply_hmat(alpha, self._h, B._h, ctypes.byref(o), ctypes.byref(out)))
+        return HBatch(out, self._cls)
+
+    def EstimateTwoNorm(self, theta=1.5, confidence=6.0, seed=1):
+        est = np.zeros(len(self))
+        check(_lib.load().dmhm_hbatch_estimate_two_norm(self._h, theta, confidence, seed,
+                                                         est.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return est
+
+    def SchulzInvert(self, numIterations=-1, theta=1.5, confidence=6.0, stopTol=1e-4, maxIterations=30,
+                     oversample=4, tol_mode=0, rel_tol=1e-10, seed=1, stall_stop=False, graph=True,
+                     raise_on_noconv=False):
+        """Every member := its inverse by Schulz iteration, each stopping on
+        its own residual estimate. Returns the per-member reports."""
+        o = _lib.SchulzOpts(numIterations, maxIterations, theta, confidence, stopTol, 0, 1e-3,
+                            _hh_opts(oversample, tol_mode, rel_tol, seed), 1 if stall_stop else 0,
+                            1 if graph else 0, 0, 0.5, 0)
+        reps = (_lib.SchulzReport * len(self))()
+        code = _lib.load().dmhm_hbatch_schulz_invert(self._h, ctypes.byref(o), reps)
+        if code == 4 and not raise_on_noconv:
+            return list(reps)
+        check(code)
+        return list(reps)
+
+
+def check_direct_invert_program(dim, sizes, num_levels, max_rank, strong, oversample=4):
+    """Host only: lowers the direct inversion of a geometry at max_rank and
+    checks every task's bounds. Returns the program's counts."""
+    sz = np.zeros(3, dtype=np.int32)
+    sz[:len(sizes)] = sizes
+    st = np.zeros(5, dtype=np.int64)
+    check(_lib.load().dmhm_check_direct_invert_program(dim, sz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), num_levels,
+                                                        max_rank, 1 if strong else 0, oversample,
+                                                        st.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+    return dict(products=int(st[0]), plans=int(st[1]), steps=int(st[2]), leaf_inversions=int(st[3]),
+                workspace=int(st[4]))

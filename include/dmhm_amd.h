@@ -48,6 +48,7 @@ enum dmhm_precision { DMHM_F64 = 8, DMHM_F32 = 4 };
 enum dmhm_memory { DMHM_HOST = 0, DMHM_DEVICE = 1 };
 
 typedef struct dmhm_hmat_s* dmhm_hmat;
+typedef struct dmhm_hbatch_s* dmhm_hbatch;
 
 /* Library set-up: binds device `device`, creates the library stream.
  * Reference: dmhm::Initialize (src/core/global.cpp:55-94; MPI and the global
@@ -246,6 +247,10 @@ typedef struct {
                             estimate is >= bf16Tol; QR, SVD and the estimate stay
                             in FP64 arithmetic */
     double bf16Tol;
+    int start;           /* 0: X_0 = alpha A^T (the reference's start); 1: X_0 =
+                            the recursive direct inverse of A (dmhm_hmat_direct_invert
+                            with opts->hh, in the precision of the first step), so
+                            the iteration only refines the truncation error */
 } dmhm_schulz_opts;
 
 typedef struct {
@@ -265,6 +270,53 @@ typedef struct {
  * ping-pong between two preallocated buffers; no allocation after the first
  * two steps of each precision. */
 int dmhm_hmat_schulz_invert(dmhm_hmat h, const dmhm_schulz_opts* opts, dmhm_schulz_report* report);
+
+/* A := inv(A) by recursive block Gauss-Jordan elimination over the children
+ * of every diagonal NODE block (HMat2d::DirectInvert / HMat3d::DirectInvert,
+ * src/HMat2d/Invert-incl.hpp:17-125, src/HMat3d/Invert-incl.hpp; dense
+ * leaves: hmat_tools::Invert, src/hmat_tools/Invert.cpp:14-42, Gauss-Jordan
+ * with partial pivoting here). Every sub-matrix product is an H*H product
+ * with randomized compression (opts: oversample, truncation, seed; NULL:
+ * oversample 4, fixed rank maxRank). The elimination is lowered once per
+ * geometry and rank into a program of merged launches (each step's
+ * independent sub-matrix products in one launch per phase) and replayed.
+ * Replaces h's contents (standard layout, rank <= maxRank). DMHM_ESINGULAR if
+ * a pivot block has a zero pivot. */
+int dmhm_hmat_direct_invert(dmhm_hmat h, const dmhm_hh_opts* opts);
+/* The cached direct-inversion program of h's geometry and rank: number of
+ * sub-matrix products, of merged launches sequences (plans), and the host
+ * seconds it took to build (builds it if needed). No reference counterpart. */
+/* Host only (no GPU needed): lowers the direct inversion of a geometry at
+ * rank maxRank and checks that every task of the program stays inside its
+ * buffers. stats[5]: sub-matrix products, merged plans, steps, dense leaf
+ * inversions, workspace elements. DMHM_EINVAL names the first violation. No
+ * reference counterpart. */
+int dmhm_check_direct_invert_program(int dim, const int* sizes, int numLevels, int maxRank, int strong,
+                                     int oversample, long long* stats);
+int dmhm_direct_invert_program_stats(dmhm_hmat h, int oversample, long long* products, long long* plans,
+                                     double* seconds);
+
+/* Batches of independent H-matrices (no reference counterpart: the reference
+ * inverts such operators one after the other, HMat3d::SchulzInvert per
+ * operator, src/HMat3d/Invert-incl.hpp:127-188). Members may differ in
+ * dimension, grid, levels and rank but share a precision; each batch
+ * operation runs every member's work in one launch per phase. A member's
+ * result is the result of the same operation on that member alone (same
+ * seeds), up to the tile engine each merged phase is tuned to. */
+int dmhm_hbatch_create(int count, const dmhm_hmat* members, dmhm_hbatch* out);   /* copies the members */
+int dmhm_hbatch_destroy(dmhm_hbatch b);
+int dmhm_hbatch_count(dmhm_hbatch b, int* count);
+/* A new handle holding a copy of member `index`. */
+int dmhm_hbatch_member(dmhm_hbatch b, int index, dmhm_hmat* out);
+/* C_i := alpha A_i B_i for every member (dmhm_hmat_multiply_hmat per member). */
+int dmhm_hbatch_multiply_hmat(double alpha, dmhm_hbatch A, dmhm_hbatch B, const dmhm_hh_opts* opts, dmhm_hbatch* C);
+/* EstimateTwoNorm of every member (estimates: count doubles). */
+int dmhm_hbatch_estimate_two_norm(dmhm_hbatch b, double theta, double confidence, uint64_t seed, double* estimates);
+/* SchulzInvert of every member in place (dmhm_hmat_schulz_invert per member;
+ * each member stops on its own estimate; FP32 or FP64 as stored, start
+ * alpha A^T, no rank model; reports: count entries). DMHM_ENOCONV if a
+ * member did not reach stopTol. */
+int dmhm_hbatch_schulz_invert(dmhm_hbatch b, const dmhm_schulz_opts* opts, dmhm_schulz_report* reports);
 
 /* Packed byte image of an H-matrix (HMat2d::PackedSize / Pack / Unpack and
  * the HMat2d(const Vector<byte>&) constructor; include/dmhm/hmat2d.hpp:89,108-112,

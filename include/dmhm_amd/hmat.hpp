@@ -252,6 +252,15 @@ public:
         detail::check(dmhm_hmat_multiply_hmat_update(alpha, need(), B.need(), beta, C.need(), &o));
     }
     // A :~= inv(A) by Schulz iteration (src/HMat2d/Invert-incl.hpp:128-196).
+    // A := inv(A) by recursive Schur complements (include/dmhm/hmat2d.hpp,
+    // src/HMat2d/Invert-incl.hpp:17-125); sub-matrix products use the H*H
+    // options of this handle's class (oversample, fixed rank).
+    void DirectInvert()
+    {
+        const dmhm_hh_opts o = hh_options();
+        detail::check(dmhm_hmat_direct_invert(need(), &o));
+    }
+
     // numIterations <= 0 stops on the estimate of ||I - X A||_2 < stopTol or
     // after 30 iterations, as the reference does; not reaching stopTol is
     // not an error (LastSchulzReport() holds the residual history).
@@ -286,6 +295,7 @@ public:
         o.graph = 1;
         o.bf16 = 0;
         o.bf16Tol = 0.5;
+        o.start = 0;
         return o;
     }
     const dmhm_schulz_report& LastSchulzReport() const { return report_; }

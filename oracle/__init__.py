@@ -82,6 +82,8 @@ def _bind(L):
             "orc_hmat_multiply": (_vp, [ctypes.c_double, _vp, _vp]),
             "orc_hmat_multiply_update": (None, [ctypes.c_double, _vp, _vp, ctypes.c_double, _vp]),
             "orc_hmat_copy": (_vp, [_vp]),
+            "orc_hmat_direct_invert": (None, [_vp]),
+            "orc_invert": (ctypes.c_int, [ctypes.c_int, _dp]),
             "orc_hmat_transpose": (_vp, [_vp]),
             "orc_hmat_scale": (None, [_vp, ctypes.c_double]),
             "orc_hmat_add_diag": (None, [_vp, ctypes.c_double]),
@@ -205,6 +207,14 @@ def rounded_update(max_rank, alpha, AU, AV, beta, BU, BV):
     return U[:m * r].reshape((m, r), order="F"), V[:n * r].reshape((n, r), order="F")
 
 
+def invert(A):
+    """Inverse of a square matrix by the oracle's Gauss-Jordan (partial pivoting)."""
+    A = np.array(A, dtype=np.float64, order="F")
+    if lib().orc_invert(A.shape[0], _d(A)) != 0:
+        raise np.linalg.LinAlgError("singular matrix")
+    return A
+
+
 def compress_dense(max_rank, D):
     D = np.asfortranarray(D, dtype=np.float64)
     m, n = D.shape
@@ -293,6 +303,11 @@ class HMat:
 
     def copy(self):
         return HMat(self.L.orc_hmat_copy(self.ptr), self.L)
+
+    def direct_invert(self):
+        """self := inv(self) by recursive block Gauss-Jordan (DirectInvert)."""
+        self.L.orc_hmat_direct_invert(self.ptr)
+        return self
 
     def transpose(self):
         return HMat(self.L.orc_hmat_transpose(self.ptr), self.L)

@@ -123,6 +123,16 @@ void* orc_hmat_multiply(double alpha, void* A, void* B)
 }
 void orc_hmat_multiply_update(double alpha, void* A, void* B, double beta, void* C)
 { HMat::multiply_update(alpha, *static_cast<HMat*>(A), *static_cast<HMat*>(B), beta, *static_cast<HMat*>(C)); }
+void orc_hmat_direct_invert(void* h) { static_cast<HMat*>(h)->direct_invert(); }
+int orc_invert(int n, double* A)
+{
+    try {
+        invert(n, A, n);
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
 void* orc_hmat_copy(void* h) { auto* C = new HMat(); C->copy_from(*static_cast<HMat*>(h)); return C; }
 void* orc_hmat_transpose(void* h) { auto* C = new HMat(); C->transpose_from(*static_cast<HMat*>(h)); return C; }
 void orc_hmat_scale(void* h, double a) { static_cast<HMat*>(h)->scale(a); }

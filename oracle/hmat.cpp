@@ -681,6 +681,50 @@ double estimate_two_norm(const HMat& A, double theta, double confidence)
     return estimate;
 }
 
+void HMat::direct_invert()
+{   // Invert-incl.hpp:17-125: Gauss-Jordan over the children of a NODE, the
+    // inverse formed in *this (starting as the identity), B a working copy.
+    if (height() != width()) throw std::logic_error("Cannot invert non-square matrices");
+    switch (type) {
+    case NODE: {
+        HMat B;
+        B.copy_from(*this);
+        set_to_identity();
+        const int nc = this->nc();
+        for (int l = 0; l < nc; ++l) {
+            child(l, l).copy_from(B.child(l, l));
+            child(l, l).direct_invert();
+            for (int j = 0; j < l; ++j) {          // A_lj := A_ll A_lj
+                HMat C;
+                C.copy_from(child(l, j));
+                multiply(1.0, child(l, l), C, child(l, j));
+            }
+            for (int j = l + 1; j < nc; ++j) {     // B_lj := A_ll B_lj
+                HMat C;
+                C.copy_from(B.child(l, j));
+                multiply(1.0, child(l, l), C, B.child(l, j));
+            }
+            for (int i = l + 1; i < nc; ++i) {
+                for (int j = 0; j <= l; ++j)       // A_ij -= B_il A_lj
+                    multiply_update(-1.0, B.child(i, l), child(l, j), 1.0, child(i, j));
+                for (int j = l + 1; j < nc; ++j)   // B_ij -= B_il B_lj
+                    multiply_update(-1.0, B.child(i, l), B.child(l, j), 1.0, B.child(i, j));
+            }
+        }
+        for (int l = nc - 1; l >= 0; --l)
+            for (int i = l - 1; i >= 0; --i)
+                for (int j = 0; j < nc; ++j)       // A_ij -= B_il A_lj
+                    multiply_update(-1.0, B.child(i, l), child(l, j), 1.0, child(i, j));
+        break;
+    }
+    case DENSE:
+        invert(D.m, D.a.data(), D.m);
+        break;
+    case LOW_RANK:
+        throw std::logic_error("Cannot invert low-rank matrices");
+    }
+}
+
 int HMat::schulz_invert(int numIterations, double theta, double confidence, double stopTol,
                         int maxIter, std::vector<double>* history)
 {   // Invert-incl.hpp:128-196

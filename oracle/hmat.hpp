@@ -18,6 +18,9 @@
 //  randomized H*H sketch ..... exact when the product rank <= r; HMT bound
 //  H*H product ............... dense product of the expanded operands
 //  EstimateTwoNorm, Schulz ... dense 2-norms, quadratic convergence
+//  DirectInvert .............. the dense inverse (numpy.linalg.inv) when no
+//                              block is truncated; the residual otherwise
+//  dense invert .............. numpy.linalg.inv
 //  LCG / Box-Muller .......... Python big-integer closed form
 // Parity unpinned (no reference build: MPI is absent; the reference holds no
 // golden vectors for these): bitwise identity with the reference of the
@@ -101,6 +104,9 @@ public:
     // Only the child C(t, s) of C := alpha A B (same conditions; timing sample).
     static void multiply_child(double alpha, const HMat& A, const HMat& B, HMat& C, int t, int s);
 
+    // A := inv(A) by recursive block Gauss-Jordan elimination over the
+    // children (DirectInvert, Invert-incl.hpp:17-125).
+    void direct_invert();
     // Schulz inversion of *this (Invert-incl.hpp:128-196). history receives the
     // per-iteration residual estimates; returns the number of iterations run.
     int schulz_invert(int numIterations, double theta, double confidence,

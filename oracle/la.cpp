@@ -32,19 +32,26 @@ void gemm(char ta, char tb, int m, int n, int k, double alpha,
 }
 
 // Generate an elementary reflector for x = [alpha; x(1:)] (dlarfg convention).
+// The column is scaled by its largest magnitude first (as dlarfg rescales
+// tiny columns): squares of entries below ~1e-154 would otherwise fall into
+// the subnormal range, and the reflector would lose its orthogonality.
 static void make_reflector(int len, double* x, double& tau)
 {
     if (len <= 1) { tau = 0.0; return; }
+    double amax = 0.0;
+    for (int i = 0; i < len; ++i) amax = std::max(amax, std::fabs(x[i]));
+    if (amax == 0.0) { tau = 0.0; return; }
+    const double inv = 1.0 / amax;
     double xnorm2 = 0.0;
-    for (int i = 1; i < len; ++i) xnorm2 += x[i] * x[i];
+    for (int i = 1; i < len; ++i) xnorm2 += (x[i] * inv) * (x[i] * inv);
     if (xnorm2 == 0.0) { tau = 0.0; return; }
-    const double alpha = x[0];
+    const double alpha = x[0] * inv;
     double beta = std::sqrt(alpha * alpha + xnorm2);
     if (alpha > 0) beta = -beta;
     tau = (beta - alpha) / beta;
-    const double scal = 1.0 / (alpha - beta);
+    const double scal = inv / (alpha - beta);
     for (int i = 1; i < len; ++i) x[i] *= scal;
-    x[0] = beta;
+    x[0] = beta * amax;
 }
 
 // A(j:m, c0:c1) := (I - tau v v^T) A(j:m, c0:c1), v = [1; A(j+1:m, j)].
@@ -195,6 +202,40 @@ void svd(int m, int n, const double* A, int lda, double* U, double* s, double* V
             }
         }
     }
+}
+
+void invert(int n, double* A, int lda)
+{
+    // Step k: bring the largest |a(i,k)|, i >= k, into row k; then
+    // a(k,:) /= a(k,k) with a(k,k) := 1/a(k,k), and for i != k
+    // a(i,:) -= a(i,k) a(k,:) with a(i,k) := -a(i,k)/a(k,k). The row swaps
+    // are undone as column swaps in reverse order.
+    std::vector<int> piv(n);
+    auto a = [&](int i, int j) -> double& { return A[i + (size_t)j * lda]; };
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(a(i, k)) > std::fabs(a(p, k))) p = i;
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < n; ++j) std::swap(a(k, j), a(p, j));
+        const double d = a(k, k);
+        if (d == 0.0) throw std::runtime_error("singular matrix");
+        const double inv = 1.0 / d;
+        for (int j = 0; j < n; ++j) a(k, j) *= inv;
+        a(k, k) = inv;
+        for (int i = 0; i < n; ++i) {
+            if (i == k) continue;
+            const double f = a(i, k);
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; ++j)
+                if (j != k) a(i, j) -= f * a(k, j);
+            a(i, k) = -f * inv;
+        }
+    }
+    for (int k = n - 1; k >= 0; --k)
+        if (piv[k] != k)
+            for (int i = 0; i < n; ++i) std::swap(a(i, k), a(i, piv[k]));
 }
 
 } // namespace orc

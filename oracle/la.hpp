@@ -49,6 +49,11 @@ void gemm(char ta, char tb, int m, int n, int k, double alpha,
           const double* A, int lda, const double* B, int ldb,
           double beta, double* C, int ldc);
 
+// In-place inverse of an n x n matrix (hmat_tools::Invert on a Dense,
+// src/hmat_tools/Invert.cpp:14-42, there LAPACK getrf + getri): Gauss-Jordan
+// elimination with partial pivoting. Throws on a zero pivot.
+void invert(int n, double* A, int lda);
+
 // Householder QR in place (geqrf semantics: R in the upper triangle,
 // reflectors below with v(0)=1 implicit, tau per reflector).
 void qr(int m, int n, double* A, int lda, double* tau);

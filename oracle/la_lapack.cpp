@@ -3,7 +3,7 @@
 // The dense routines of la.hpp bound to a Fortran BLAS/LAPACK (LP64), as the
 // reference binds them (include/dmhm/core/blas.hpp:25-200,
 // include/dmhm/core/lapack.hpp:21-360): dgemm, dgeqrf, dgeqp3, dorgqr,
-// dormqr, dgesvd. Built into liboracle_lapack.so (with MKL's sequential
+// dormqr, dgesvd, dgetrf + dgetri (hmat_tools::Invert). Built into liboracle_lapack.so (with MKL's sequential
 // layer) for the timed CPU baseline; tests check it against the
 // self-contained liboracle.so on the same inputs.
 #include "la.hpp"
@@ -21,6 +21,8 @@ void dorgqr_(const int* m, const int* n, const int* k, double* A, const int* lda
 void dormqr_(const char* side, const char* trans, const int* m, const int* n, const int* k, const double* A,
              const int* lda, const double* tau, double* C, const int* ldc, double* work, const int* lwork,
              int* info);
+void dgetrf_(const int* m, const int* n, double* A, const int* lda, int* ipiv, int* info);
+void dgetri_(const int* n, double* A, const int* lda, const int* ipiv, double* work, const int* lwork, int* info);
 void dgesvd_(const char* jobu, const char* jobvt, const int* m, const int* n, double* A, const int* lda,
              double* s, double* U, const int* ldu, double* VT, const int* ldvt, double* work, const int* lwork,
              int* info);
@@ -100,6 +102,17 @@ void svd(int m, int n, const double* A, int lda, double* U, double* s, double* V
     with_work([&](double* w, int* lw, int* info) {
         dgesvd_(&job, &job, &m, &n, a.data(), &lda2, s, U, &ldu, VT, &ldvt, w, lw, info);
     }, "dgesvd");
+}
+
+void invert(int n, double* A, int lda)
+{
+    if (n == 0) return;
+    std::vector<int> ipiv(n);
+    int info = 0;
+    dgetrf_(&n, &n, A, &lda, ipiv.data(), &info);
+    if (info > 0) throw std::runtime_error("singular matrix");
+    check(info, "dgetrf");
+    with_work([&](double* w, int* lw, int* inf) { dgetri_(&n, A, &lda, ipiv.data(), w, lw, inf); }, "dgetri");
 }
 
 } // namespace orc

@@ -158,6 +158,14 @@ static int run_gpu()
     expect(e < 1e-3, "SchulzInvert: X (H x) ~= x", e);
     expect(rep.iterations > 0, "SchulzInvert report", rep.residualEstimate);
 
+    // DirectInvert (tests/HMat2d/Invert.cpp inverts the same way): X (H x) ~= x.
+    HMat2d<double> XD;
+    XD.CopyFrom(H);
+    XD.DirectInvert();
+    XD.Multiply(1.0, y, z);
+    e = rel_diff(z, x, m);
+    expect(e < 1e-3, "DirectInvert: X (H x) ~= x", e);
+
     // LowRank constructor: H = U V^T exactly (tests/HMat2d/LowRankConstructor.cpp).
     LowRank<double> F;
     F.U.Resize(m, 2);

@@ -219,3 +219,55 @@ def test_set_to_random_fills_rank_max():
     assert st["max_rank"] == 5
     d, f = H.count_blocks(4)
     assert st["sum_rank"] == 5 * f.sum()
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 33, 64, 70])
+def test_dense_invert_matches_numpy(n):
+    """Gauss-Jordan with partial pivoting (the oracle's hmat_tools::Invert)
+    against LAPACK's inverse, including a matrix whose leading entries are
+    zero (pivoting is required) and a permutation."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + n * np.eye(n)
+    np.testing.assert_allclose(orc.invert(A), np.linalg.inv(A), rtol=0, atol=1e-12 * np.abs(np.linalg.inv(A)).max())
+    if n > 1:
+        P = np.eye(n)[rng.permutation(n)]
+        P[:, [0, 1]] = P[:, [1, 0]] if P[0, 0] else P[:, [0, 1]]
+        B = P @ (np.eye(n) + 0.1 * rng.standard_normal((n, n)))
+        B[0, 0] = 0.0
+        np.testing.assert_allclose(orc.invert(B), np.linalg.inv(B), rtol=0,
+                                   atol=1e-11 * np.abs(np.linalg.inv(B)).max())
+
+
+def test_dense_invert_rejects_singular():
+    with pytest.raises(Exception):
+        orc.invert(np.ones((3, 3)))
+
+
+@pytest.mark.parametrize("n,L,dim,strong,rank", [(16, 3, 2, True, 16), (16, 3, 2, False, 64), (8, 3, 3, True, 8),
+                                                 (12, 3, 2, True, 36)])
+def test_direct_invert_exact_without_truncation(n, L, dim, strong, rank):
+    """With maxRank at least every admissible block's dimension nothing is
+    truncated, so the recursive Gauss-Jordan gives the inverse of the dense
+    matrix (a dropped elimination term, a wrong sign or a transposed operand
+    would not)."""
+    op, N, args = _op(n, L, dim, smooth=False, seed=3)
+    H = orc.HMat.from_csr(dim, op["sizes"], L, rank, strong, *args)
+    D = W.csr_to_dense(*args, N)
+    X = H.copy().direct_invert().to_dense()
+    Xr = np.linalg.inv(D)
+    assert np.abs(X - Xr).max() < 1e-10 * np.abs(Xr).max()
+
+
+def test_direct_invert_truncated_residual_falls_with_rank():
+    """Truncated blocks leave a residual ||I - X A||_2 that shrinks as maxRank
+    grows (smooth operator: its inverse's admissible blocks are numerically
+    low rank)."""
+    op, N, args = _op(32, 4, 2)
+    D = W.csr_to_dense(*args, N)
+    res = []
+    for rank in (2, 4, 8):
+        H = orc.HMat.from_csr(2, op["sizes"], 4, rank, True, *args)
+        X = H.copy().direct_invert().to_dense()
+        res.append(np.linalg.norm(np.eye(N) - X @ D, 2))
+    assert res[0] > res[1] > res[2]
+    assert res[2] < 1e-3
