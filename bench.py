@@ -1,14 +1,16 @@
 #!/usr/bin/env python
 """Benchmark of the H-matrix hot path on one MI355X.
 
-Workload (one "step"): Schulz inversion X_{k+1} = (2I - X_k A) X_k from
-X_0 = alpha A^T of the 2D 256^2 operator -div(a grad u) + u with lognormal
-coefficient (log a Gaussian random field, sigma = 1, correlation length 0.1,
-seed 0), HMat2d numLevels = 6, maxRank = 16, strong admissibility, run until
-the estimate of ||I - X A||_2 drops below 1e-8 or stops decreasing (the
-rank-16 truncation floor). The timed form is the mixed one (FP32 iterations
-until the estimate is below 1e-3, FP64 for the rest); the plain FP64 form is
-timed once beside it. Also reported: H*X on 1 and 64 FP32 right-hand sides
+Workload (one "step"): the inverse of the 2D 256^2 operator -div(a grad u) + u
+with lognormal coefficient (log a Gaussian random field, sigma = 1,
+correlation length 0.1, seed 0), HMat2d numLevels = 6, maxRank = 16, strong
+admissibility, to an estimate of ||I - X A||_2 below 1e-8 or the rank floor
+(the estimate stops decreasing). The timed form starts the Schulz iteration
+X_{k+1} = (2I - X_k A) X_k from the recursive direct inversion
+(HMat2d::DirectInvert) and refines in FP64. Timed once beside it: the direct
+inversion alone, the mixed form from the direct start, and the reference's own
+start X_0 = alpha A^T in plain FP64 and mixed (FP32 iterations until the
+estimate is below 1e-3). Also reported: H*X on 1 and 64 FP32 right-hand sides
 (HBM GB/s, roofline of the accumulation launch), one steady-state H*H product
 (TFLOP/s), and the FP64 CPU oracle on the host cores.
 
@@ -146,8 +148,9 @@ def main():
     ap.add_argument("--switch-tol", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--mode", choices=["full", "matvec", "hh"], default="full",
-                    help="matvec / hh: only the kernel loop (for rocprofv3 runs)")
+    ap.add_argument("--mode", choices=["full", "matvec", "hh", "inverse"], default="full",
+                    help="matvec / hh / inverse: only that loop (for rocprofv3 runs); inverse: the timed "
+                         "inversion of the full mode, --steps times")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--config", choices=["c2", "c4"], default="c2",
                     help="c2: 2D 256^2 (the headline workload); c4: 2D 512^2, sigma=1.5, l=0.05, "
@@ -201,46 +204,88 @@ def main():
         print(json.dumps(out))
         return
 
-    # Warm-up: plan building and workspace. Two iterations in each precision
-    # reach every plan of the loop (first product with the imported operator,
-    # then products of full-rank iterates).
+    sync = dmhm_amd.load().dmhm_synchronize
+    if a.mode == "inverse":
+        for w in range(a.warmup):
+            X = A.Copy()
+            X.SchulzInvert(start="direct", **sopts)
+        sync()
+        t = time.perf_counter()
+        for s in range(a.steps):
+            X = A.Copy()
+            rep = X.SchulzInvert(start="direct", **sopts)
+        sync()
+        print(json.dumps({"mode": "inverse", "config": a.config, "seconds": (time.perf_counter() - t) / a.steps,
+                          "iterations": rep.iterations, "estimate": rep.residualEstimate}))
+        return
+    dstats = A.direct_invert_program()     # builds (and caches) the direct-inversion program
+    log("bench: direct-inversion program: %d sub-matrix products in %d merged plans, built in %.2f s"
+        % dstats)
+    # Warm-up: plans, workspace and tile-engine tuning of every form timed below.
     for w in range(a.warmup):
         log("bench: warmup %d" % w)
         X = A.Copy()
-        X.SchulzInvert(numIterations=2)
-        X = A.Copy()
-        X.SchulzInvert(numIterations=2, mixed=True)
-    dmhm_amd.load().dmhm_synchronize()
+        X.SchulzInvert(numIterations=2, start="direct")
+        if w == 0:
+            for mixed in (False, True):
+                X = A.Copy()
+                X.SchulzInvert(numIterations=2, mixed=mixed)
+            X = A.Copy()
+            X.SchulzInvert(numIterations=2, mixed=True, start="direct")
+    sync()
 
-    log("bench: timing %d mixed Schulz inversion(s)" % a.steps)
+    # The timed step: the inverse to ||I - X A||_2 < stopTol (or the rank
+    # floor): X_0 from the direct inversion, refined by the Schulz iteration
+    # in FP64.
+    log("bench: timing %d inversion(s) (direct start, FP64 Schulz refinement)" % a.steps)
     t0 = time.perf_counter()
     reps = []
     for s in range(a.steps):
         X = A.Copy()
-        reps.append(X.SchulzInvert(mixed=True, **sopts))
-    dmhm_amd.load().dmhm_synchronize()
-    t1 = time.perf_counter()
-    sec_mixed = (t1 - t0) / a.steps
+        reps.append(X.SchulzInvert(start="direct", **sopts))
+    sync()
+    sec = (time.perf_counter() - t0) / a.steps
     rep = reps[-1]
-    res_mixed = residual_power_estimate(A, X, n)
-    log("bench: mixed %.2f s, %d iterations (%d FP32), estimate %.3e, power check %.3e"
-        % (sec_mixed, rep.iterations, rep.iterationsLowPrecision, rep.residualEstimate, res_mixed))
+    res = residual_power_estimate(A, X, n)
+    log("bench: direct start %.3f s, %d iterations, estimate %.3e, power check %.3e"
+        % (sec, rep.iterations, rep.residualEstimate, res))
 
-    sec64 = None
-    rep64 = None
-    res64 = None
-    if not a.no_fp64:
-        log("bench: timing the FP64 Schulz inversion")
-        X64 = A.Copy()
+    def timed(**kw):
+        Y = A.Copy()
+        sync()
         t = time.perf_counter()
-        rep64 = X64.SchulzInvert(mixed=False, **sopts)
-        dmhm_amd.load().dmhm_synchronize()
-        sec64 = time.perf_counter() - t
+        r_ = Y.SchulzInvert(**kw)
+        sync()
+        return Y, r_, time.perf_counter() - t
+
+    # The direct inversion alone.
+    Xd = A.Copy()
+    sync()
+    t = time.perf_counter()
+    Xd.DirectInvert()
+    sync()
+    sec_direct = time.perf_counter() - t
+    res_direct = residual_power_estimate(A, Xd, n)
+    log("bench: direct inversion alone %.3f s, power check %.3e" % (sec_direct, res_direct))
+    # The mixed form from the direct start (X_0 from an FP32 direct inversion).
+    Xm, repm, sec_mixed = timed(start="direct", mixed=True, **sopts)
+    res_mixed = residual_power_estimate(A, Xm, n)
+    log("bench: direct start mixed %.3f s, %d iterations (%d FP32), estimate %.3e, power check %.3e"
+        % (sec_mixed, repm.iterations, repm.iterationsLowPrecision, repm.residualEstimate, res_mixed))
+    # The reference's own start X_0 = alpha A^T, plain FP64 and mixed.
+    sec64 = rep64 = res64 = sec_ref_mixed = rep_ref_mixed = res_ref_mixed = None
+    if not a.no_fp64:
+        X64, rep64, sec64 = timed(mixed=False, **sopts)
         res64 = residual_power_estimate(A, X64, n)
-        log("bench: fp64 %.2f s, %d iterations, estimate %.3e, power check %.3e"
+        log("bench: reference start FP64 %.2f s, %d iterations, estimate %.3e, power check %.3e"
             % (sec64, rep64.iterations, rep64.residualEstimate, res64))
+        Xr, rep_ref_mixed, sec_ref_mixed = timed(mixed=True, **sopts)
+        res_ref_mixed = residual_power_estimate(A, Xr, n)
+        log("bench: reference start mixed %.2f s, %d iterations (%d FP32), estimate %.3e, power check %.3e"
+            % (sec_ref_mixed, rep_ref_mixed.iterations, rep_ref_mixed.iterationsLowPrecision,
+               rep_ref_mixed.residualEstimate, res_ref_mixed))
         v = np.random.default_rng(9).standard_normal(n)
-        diff = np.linalg.norm(X.Multiply(1.0, v) - X64.Multiply(1.0, v)) / np.linalg.norm(X64.Multiply(1.0, v))
+        diff = np.linalg.norm(X.Multiply(1.0, v) - X64.Multiply(1.0, v)) / np.linalg.norm(X.Multiply(1.0, v))
     else:
         diff = None
 
@@ -276,35 +321,48 @@ def main():
     cpu = None
     if not a.no_cpu:
         log("bench: CPU oracle sample (steady-state iteration, one core)")
-        cpu = cpu_baseline(cpu_baseline_sample(cfg), rep64.iterations if rep64 is not None else rep.iterations)
+        cpu = cpu_baseline(cpu_baseline_sample(cfg), rep64.iterations if rep64 is not None else 43)
         log("bench: cpu %.0f s extrapolated" % cpu["value"])
 
     out = {
         "metric": METRIC,
-        "value": sec_mixed,
+        "value": sec,
         "unit": "s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": sec_mixed * 1e3,
+        "ms_per_step": sec * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "2D 256^2 -div(a grad u)+u, log a GRF sigma=1 l=0.1 (seed 0), HMat2d numLevels=6 "
-                               "maxRank=16 strong; Schulz to ||I-XA||<1e-8 or truncation floor, FP32 iterations "
-                               "then FP64 (switch at 1e-3)",
+                               "maxRank=16 strong; inverse to ||I-XA||<1e-8 or the rank floor: X_0 by the "
+                               "recursive direct inversion, refined by FP64 Schulz steps",
                    "n": n, "numLevels": L, "maxRank": r, "oversample": 4, "stopTol": a.stop_tol},
-        "schulz": {"mixed_seconds": sec_mixed, "mixed_iterations": rep.iterations,
-                   "mixed_fp32_iterations": rep.iterationsLowPrecision,
-                   "mixed_residual_estimate": rep.residualEstimate, "mixed_residual_power_check": res_mixed,
-                   "fp64_seconds": sec64, "fp64_iterations": rep64.iterations if rep64 else None,
-                   "fp64_residual_estimate": rep64.residualEstimate if rep64 else None,
-                   "fp64_residual_power_check": res64,
-                   "speedup_mixed_vs_fp64": (sec64 / sec_mixed) if sec64 else None,
-                   "mixed_vs_fp64_relative_difference": diff,
-                   "reached_1e-8": bool(rep.residualEstimate < 1e-8)},
+        "schulz": {"direct_start_seconds": sec, "direct_start_iterations": rep.iterations,
+                   "direct_start_residual_estimate": rep.residualEstimate,
+                   "direct_start_residual_power_check": res,
+                   "reached_1e-8": bool(rep.residualEstimate < 1e-8),
+                   "reached_1e-8_power_check": bool(res < 1e-8),
+                   "direct_invert_seconds": sec_direct, "direct_invert_residual_power_check": res_direct,
+                   "direct_invert_products": dstats[0], "direct_invert_merged_plans": dstats[1],
+                   "direct_start_mixed_seconds": sec_mixed, "direct_start_mixed_iterations": repm.iterations,
+                   "direct_start_mixed_fp32_iterations": repm.iterationsLowPrecision,
+                   "direct_start_mixed_residual_estimate": repm.residualEstimate,
+                   "direct_start_mixed_residual_power_check": res_mixed,
+                   "reference_start_fp64_seconds": sec64,
+                   "reference_start_fp64_iterations": rep64.iterations if rep64 else None,
+                   "reference_start_fp64_residual_estimate": rep64.residualEstimate if rep64 else None,
+                   "reference_start_fp64_residual_power_check": res64,
+                   "reference_start_mixed_seconds": sec_ref_mixed,
+                   "reference_start_mixed_iterations": rep_ref_mixed.iterations if rep_ref_mixed else None,
+                   "reference_start_mixed_residual_estimate":
+                       rep_ref_mixed.residualEstimate if rep_ref_mixed else None,
+                   "reference_start_mixed_residual_power_check": res_ref_mixed,
+                   "speedup_vs_reference_start_fp64": (sec64 / sec) if sec64 else None,
+                   "direct_vs_reference_fp64_relative_difference": diff},
         "matvec": {"f32_k1_ms": ms1[0], "f32_k1_GBps": gbs1, "f32_k1_accumulation_GBps": acc_gbs,
                    "f32_k64_ms": ms64[0], "f32_k64_GBps": gbs64, "f32_k64_TFLOPs": tflops64,
                    "f64_k1_ms": msd[0], "f64_k1_GBps": gbsd, "f32_k1_bytes": by1.sum()},

@@ -123,6 +123,7 @@ SIGNATURES = {
     "dmhm_hmat_packed_size": (ctypes.c_int, [_h, ctypes.POINTER(ctypes.c_uint64)]),
     "dmhm_hmat_pack": (ctypes.c_int, [_h, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     "dmhm_hmat_unpack": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _hp]),
+    "dmhm_hmat_unpack_image": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), _hp]),
     "dmhm_hmat_copy": (ctypes.c_int, [_h, _hp]),
     "dmhm_hmat_transpose": (ctypes.c_int, [_h, _hp]),
     "dmhm_hmat_scale": (ctypes.c_int, [_h, ctypes.c_double]),

@@ -123,7 +123,7 @@ static void build_parallel(size_t count, const std::function<void(size_t)>& buil
 static Plan* merged_product(const Arena& A, const Arena& B, const Arena& C, int oversample)
 {
     const size_t M = A.size();
-    struct Cls { const dmhm_hmat_s *a, *b, *c; std::unique_ptr<Plan> plan; };
+    struct Cls { const dmhm_hmat_s *a, *b, *c; std::shared_ptr<Plan> plan; };
     std::vector<Cls> cls;
     std::vector<size_t> of(M);
     for (size_t m = 0; m < M; ++m) {
@@ -155,11 +155,18 @@ static Plan* merged_product(const Arena& A, const Arena& B, const Arena& C, int 
         p.val[SLOT_B] = B.val[m]; p.rank[SLOT_B] = B.rank[m];
         p.val[SLOT_C] = C.val[m]; p.rank[SLOT_C] = C.rank[m];
     }
-    std::unique_ptr<Plan> merged(merge_plans(parts));
+    // Tile phases by segment: members of a class share their class plan's
+    // tile arrays on the device (no copies); the smaller task lists (QR,
+    // SVD, sketches) are concatenated.
+    std::unique_ptr<Plan> merged(merge_plans(parts, true));
     const int64_t slots[NUM_SLOTS] = {A.totalVals, B.totalVals, C.totalVals, 0, 0, 0, 0, 0};
     const int64_t ranks[NUM_SLOTS] = {A.totalRanks, B.totalRanks, C.totalRanks, 0, 0, 0, 0, 0};
     const std::string e = check_plan(*merged, slots, ranks);
     if (!e.empty()) throw Error(DMHM_EINVAL, "merged batch product: " + e);
+    for (auto& x : cls) {
+        x.plan->upload();
+        merged->members.push_back(x.plan);
+    }
     merged->upload();
     return merged.release();
 }
@@ -214,7 +221,6 @@ using namespace dmhm;
 
 struct dmhm_hbatch_s {
     Arena a;                                            // the members
-    std::map<std::string, std::unique_ptr<Plan>> plans; // merged plans by role and layouts
     // Power iteration: per-member vector segments and buffers, log accumulators.
     DeviceBuffer segs, vecs, logAcc;
     std::vector<VecSegment> hsegs;
@@ -225,7 +231,6 @@ struct dmhm_hbatch_s {
     DeviceBuffer diagOff, diagSize;
     int diagCount = 0;
     ~dmhm_hbatch_s() {
-        plans.clear();
         segs.release(); vecs.release(); logAcc.release();
         diagOff.release(); diagSize.release();
         if (hRead) (void)hipHostFree(hRead);
@@ -300,20 +305,43 @@ static void read_batch_two_norm(dmhm_hbatch_s& B, double theta, std::vector<doub
     for (size_t m = 0; m < M; ++m) est[m] = theta * std::exp(B.hRead[m]);
 }
 
+// Merged plans are cached by the member layouts they were built for (layout
+// ids are unique in the process and layouts live as long as a matrix uses
+// them), so batches of the same operators share them.
+// (Allocated once and released by dmhm_finalize, not by a static destructor
+// that could run after the HIP runtime is gone.)
+static std::map<std::string, std::unique_ptr<Plan>>* g_batchPlans = new std::map<std::string, std::unique_ptr<Plan>>();
+
+void release_batch_plans() { g_batchPlans->clear(); }
+
 static Plan& batch_plan(dmhm_hbatch_s& B, const std::string& key, const std::function<Plan*()>& make)
 {
-    auto it = B.plans.find(key);
-    if (it != B.plans.end()) return *it->second;
+    (void)B;
+    auto it = g_batchPlans->find(key);
+    if (it != g_batchPlans->end()) return *it->second;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHECK(hipStreamIsCapturing(g_lib.stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        throw Error(DMHM_EINVAL, "a batch plan would be built while the step is captured");
     std::unique_ptr<Plan> p(make());
     Plan& ref = *p;
-    B.plans[key] = std::move(p);
+    (*g_batchPlans)[key] = std::move(p);
     return ref;
 }
 
-static std::string arena_key(const Arena& a)
+// Identity of an arena's members for the plan cache: geometry, layout id
+// (standard layouts are numbered by rank, so the geometry is part of it) and
+// the full-rank flag, member by member.
+// (The flag matters for operands only: it selects their plan form.)
+static std::string arena_key(const Arena& a, bool withFull = true)
 {
     std::string k;
-    for (size_t i = 0; i < a.size(); ++i) k += std::to_string(a.at(i)->lay->id) + (a.at(i)->full ? "f," : ",");
+    for (size_t i = 0; i < a.size(); ++i) {
+        const Geometry& g = a.at(i)->geo();
+        k += std::to_string(g.dim) + "/" + std::to_string(g.sizes[0]) + "x" + std::to_string(g.sizes[1]) + "x" +
+             std::to_string(g.sizes[2]) + "/" + std::to_string(g.numLevels) + (g.strong ? "s/" : "w/") +
+             std::to_string(a.at(i)->lay->id) + (withFull && a.at(i)->full ? "f," : ",");
+    }
     return k;
 }
 
@@ -324,7 +352,8 @@ static void batch_multiply_into(dmhm_hbatch_s& P, double alpha, const Arena& A, 
     if (A.prec != B.prec || A.prec != C.prec) throw Error(DMHM_EINVAL, "operands must share a precision");
     const int over = o.oversample > 0 ? o.oversample : 4;
     if (o.rankModel) throw Error(DMHM_EINVAL, "batched products take no rank model");
-    const std::string key = "hh:" + arena_key(A) + "|" + arena_key(B) + "|" + arena_key(C) + ":" + std::to_string(over);
+    const std::string key = "hh:" + arena_key(A) + "|" + arena_key(B) + "|" + arena_key(C, false) + ":" +
+                            std::to_string(over);
     Plan& p = batch_plan(P, key, [&] { return merged_product(A, B, C, over); });
     ensure_ws(p, A.esize());
     Bases b{};
@@ -518,7 +547,6 @@ static void batch_schulz(dmhm_hbatch_s& B, const dmhm_schulz_opts& o, dmhm_schul
     std::swap(A.vals, R.vals);
     std::swap(A.ranks, R.ranks);
     for (size_t m = 0; m < M; ++m) A.at(m)->full = true;
-    B.plans.clear();
     B.diagCount = 0;
     if (!conv) throw Error(DMHM_ENOCONV, "Schulz iteration did not reach the stop tolerance in every member");
 }

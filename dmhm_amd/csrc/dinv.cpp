@@ -319,8 +319,9 @@ void run_program(DinvProgram& P, T* vals, int* ranks, int* singular, const dmhm_
     for (DinvStep& st : P.steps) {
         switch (st.kind) {
         case DinvStep::RANGES:
-            launch_copy_ranges<T>(vals, vals, tab + st.valAt, st.nVal, st.maxVal, st.fill, s);
-            launch_copy_ranges<int>(ranks, ranks, tab + st.rankAt, st.nRank, st.maxRankLen, st.fill, s);
+            // The rank ranges follow the value ranges in the table.
+            launch_arena_ranges<T>(vals, ranks, tab + st.valAt, st.nVal, st.nRank,
+                                   std::max(st.maxVal, st.maxRankLen), st.fill, s);
             break;
         case DinvStep::PLAN:
             // Each merged plan draws its own sketches.
@@ -387,20 +388,29 @@ void direct_invert(dmhm_hmat_s* A, const dmhm_hh_opts& o)
 
 // Host-only lowering and bounds check of the program of a geometry at a
 // rank (standard layout); stats: sub-matrix products, merged plans, steps,
-// dense leaf inversions, workspace elements.
+// dense leaf inversions, workspace elements, plan phases (launches), identity
+// phases among them.
 void check_direct_invert_program(const Geometry& g, int maxRank, int over, long long* stats)
 {
     std::vector<int> caps(g.lrBlocks.size(), std::max(maxRank, 1));
     const Layout L = Layout::make(g, caps, 0);
     auto P = build_program(g, L, maxRank, over);
     check_program(*P);
-    long long inv = 0;
-    for (const DinvStep& st : P->steps) inv += st.kind == DinvStep::INVERT;
+    long long inv = 0, phases = 0, eyes = 0;
+    for (const DinvStep& st : P->steps) {
+        inv += st.kind == DinvStep::INVERT;
+        if (st.kind == DinvStep::PLAN) {
+            phases += (long long)st.plan->phases.size();
+            for (const Phase& ph : st.plan->phases) eyes += ph.kind == PH_EYE;
+        }
+    }
     stats[0] = P->products;
     stats[1] = P->plans;
     stats[2] = (long long)P->steps.size();
     stats[3] = inv;
     stats[4] = P->wsElems;
+    stats[5] = phases;
+    stats[6] = eyes;
 }
 
 // Statistics of the cached program (products, merged plans, build seconds).

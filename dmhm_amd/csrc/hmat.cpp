@@ -746,8 +746,14 @@ dmhm_hmat_s* transpose_hmat(dmhm_hmat_s* s)
     }
     std::shared_ptr<Layout> Ld = s->lay;
     if (!same) {
-        std::lock_guard<std::mutex> lk(g_lib.mu);
-        Ld = std::make_shared<Layout>(Layout::make(g, caps, g_lib.nextLayoutId++));
+        // One transposed layout per source layout, so that plans over
+        // transposes (the Schulz start) are cached across calls.
+        auto& cached = s->gc->transLayouts[Ls.id];
+        if (!cached) {
+            std::lock_guard<std::mutex> lk(g_lib.mu);
+            cached = std::make_shared<Layout>(Layout::make(g, caps, g_lib.nextLayoutId++));
+        }
+        Ld = cached;
     }
     std::unique_ptr<dmhm_hmat_s> d(alloc_hmat(s->gc, Ld, s->prec, s->maxRank));
     d->full = s->full;
@@ -1438,6 +1444,7 @@ int dmhm_finalize(void)
     return guard([&] {
         if (!g_lib.ready) return;
         HIPCHECK(hipStreamSynchronize(g_lib.stream));
+        release_batch_plans();
         delete g_lib.tuneEv;
         g_lib.tuneEv = nullptr;
         g_lib.ws.release(); g_lib.stageX.release(); g_lib.stageY.release();
@@ -2136,6 +2143,42 @@ static dmhm_hmat_s* unpack_hmat(const char* in, uint64_t size)
     return h.release();
 }
 
+// Length of a packed image, from its header and rank table (the reference's
+// Unpack(const byte*) reads an image of implicit length).
+static uint64_t image_size(const char* in)
+{
+    PackHeader hd;
+    std::memcpy(&hd, in, sizeof(hd));
+    if (std::memcmp(hd.magic, kPackMagic, 8) != 0) throw Error(DMHM_EINVAL, "not a packed dmhm_amd H-matrix");
+    if (hd.precision != DMHM_F64 && hd.precision != DMHM_F32) throw Error(DMHM_EINVAL, "bad precision in image");
+    auto gc = get_geo(hd.dim, hd.sizes, hd.numLevels, hd.strong != 0);
+    const Geometry& g = gc->geo;
+    if (hd.nLowRank != (int64_t)g.lrBlocks.size() || hd.denseElems < 0)
+        throw Error(DMHM_EINVAL, "image does not match its geometry");
+    const char* rk = in + sizeof(hd) + sizeof(int32_t) * hd.nLowRank;
+    uint64_t elems = (uint64_t)hd.denseElems;
+    for (size_t i = 0; i < g.lrBlocks.size(); ++i) {
+        int32_t r = 0;
+        std::memcpy(&r, rk + sizeof(int32_t) * i, sizeof(r));
+        if (r < 0) throw Error(DMHM_EINVAL, "bad rank in image");
+        const int b = g.lrBlocks[i];
+        elems += (uint64_t)(g.tbox(b).size + g.sbox(b).size) * r;
+    }
+    const uint64_t es = hd.precision == DMHM_F64 ? 8 : 4;
+    return sizeof(hd) + 2 * sizeof(int32_t) * hd.nLowRank + elems * es;
+}
+
+int dmhm_hmat_unpack_image(const void* buf, uint64_t* consumed, dmhm_hmat* out)
+{
+    return guard([&] {
+        require_ready();
+        if (!buf || !out) throw Error(DMHM_EINVAL, "null pointer");
+        const uint64_t n = image_size(static_cast<const char*>(buf));
+        *out = unpack_hmat(static_cast<const char*>(buf), n);
+        if (consumed) *consumed = n;
+    });
+}
+
 int dmhm_hmat_packed_size(dmhm_hmat h, uint64_t* bytes)
 {
     return guard([&] {
@@ -2224,6 +2267,31 @@ namespace dmhm {
 void Plan::upload()
 {
     for (auto& ph : phases) {
+        if (!ph.segSrc.empty()) {
+            // Segment table of a merged tile phase: the members' source
+            // arrays must already be on the device.
+            std::vector<TileSeg> segs;
+            int b1 = 0, b4 = 0;
+            for (const auto& s : ph.segSrc) {
+                const int n = (int)s.src->tiles.size();
+                if (n > 0 && (!s.src->dTiles || !s.src->dContribs))
+                    throw Error(DMHM_EINVAL, "segment source plan not uploaded");
+                TileSeg t{};
+                t.tiles = static_cast<const Tile*>(s.src->dTiles);
+                t.cs = static_cast<const Contrib*>(s.src->dContribs);
+                for (int k = 0; k < NUM_SLOTS; ++k) { t.val[k] = s.val[k]; t.rank[k] = s.rank[k]; }
+                t.count = n;
+                t.first1 = b1;
+                t.first4 = b4;
+                b1 += n;
+                b4 += (n + 3) / 4;
+                segs.push_back(t);
+            }
+            ph.segBlocks1 = b1;
+            ph.segBlocks4 = b4;
+            HIPCHECK(hipMalloc(&ph.dSegs, segs.size() * sizeof(TileSeg)));
+            HIPCHECK(copy_sync(ph.dSegs, segs.data(), segs.size() * sizeof(TileSeg), hipMemcpyHostToDevice));
+        }
         if (!ph.tiles.empty()) {
             HIPCHECK(hipMalloc(&ph.dTiles, ph.tiles.size() * sizeof(Tile)));
             HIPCHECK(copy_sync(ph.dTiles, ph.tiles.data(), ph.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
@@ -2294,7 +2362,8 @@ void Plan::release()
         if (ph.dContribs) (void)hipFree(ph.dContribs);
         if (ph.dTasks) (void)hipFree(ph.dTasks);
         if (ph.dAux) (void)hipFree(ph.dAux);
-        ph.dTiles = ph.dContribs = ph.dTasks = ph.dAux = nullptr;
+        if (ph.dSegs) (void)hipFree(ph.dSegs);
+        ph.dTiles = ph.dContribs = ph.dTasks = ph.dAux = ph.dSegs = nullptr;
     }
 }
 } // namespace dmhm

@@ -19,6 +19,35 @@ __global__ void ranges_kernel(const T* __restrict__ src, T* __restrict__ dst, co
         dst[r.dst + e] = FILL ? T(0) : src[r.src + e];
 }
 
+// The value ranges (blockIdx.y < nVal) and the rank ranges of one step in
+// one launch.
+template <typename T, bool FILL>
+__global__ void ranges2_kernel(T* __restrict__ vals, int* __restrict__ ranks, const CopyRange* __restrict__ rs, int nVal)
+{
+    const int y = blockIdx.y;
+    const CopyRange r = rs[y];
+    if (y < nVal) {
+        for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < r.len; e += (int64_t)gridDim.x * blockDim.x)
+            vals[r.dst + e] = FILL ? T(0) : vals[r.src + e];
+    } else {
+        for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < r.len; e += (int64_t)gridDim.x * blockDim.x)
+            ranks[r.dst + e] = FILL ? 0 : ranks[r.src + e];
+    }
+}
+
+template <typename T>
+void launch_arena_ranges(T* vals, int* ranks, const CopyRange* ranges, int nVal, int nRank, int64_t maxLen, bool fill,
+                         hipStream_t s)
+{
+    const int count = nVal + nRank;
+    if (count <= 0 || maxLen <= 0) return;
+    if (count > 65535) throw std::invalid_argument("too many ranges in one launch");
+    const dim3 grid((unsigned)std::min<int64_t>((maxLen + 255) / 256, 2048), (unsigned)count);
+    if (fill) ranges2_kernel<T, true><<<grid, 256, 0, s>>>(vals, ranks, ranges, nVal);
+    else ranges2_kernel<T, false><<<grid, 256, 0, s>>>(vals, ranks, ranges, nVal);
+    HIP_LAUNCH_CHECK();
+}
+
 template <typename T>
 void launch_copy_ranges(const T* src, T* dst, const CopyRange* ranges, int count, int64_t maxLen, bool fill,
                         hipStream_t s)
@@ -140,6 +169,7 @@ void launch_dense_inverse(const T* src, T* dst, int m, int* singular, hipStream_
 
 #define INST(T)                                                                                          \
     template void launch_copy_ranges<T>(const T*, T*, const CopyRange*, int, int64_t, bool, hipStream_t); \
+    template void launch_arena_ranges<T>(T*, int*, const CopyRange*, int, int, int64_t, bool, hipStream_t); \
     template void launch_eye_multi<T>(T*, const int64_t*, int, int, hipStream_t);                      \
     template void launch_dense_inverse<T>(const T*, T*, int, int*, hipStream_t);
 INST(double)

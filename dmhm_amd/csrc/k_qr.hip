@@ -145,7 +145,7 @@ __device__ __forceinline__ double pick_col(const double (&a)[KMAX], int j)
     return v;
 }
 
-template <typename T, int KMAX, int NW>
+template <typename T, int KMAX, int NW, bool BATCHED>
 __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* __restrict__ tasks)
 {
     __shared__ double part[2][NW][KMAX];
@@ -159,6 +159,10 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
     if (k <= 0) return;
     const int i = threadIdx.x, w = i >> 6, lane = i & 63;
     const bool inPanel = i < m;
+    // BATCHED: all column sums of a step reduced together (lowest latency,
+    // more registers: occupancy 2), for phases of few panels; else one
+    // column at a time (more panels in flight), for large phases.
+    constexpr bool kBatchedSums = BATCHED;
     double a[KMAX];
 #pragma unroll
     for (int c = 0; c < KMAX; ++c) a[c] = (inPanel && c < k) ? (double)A[i + (int64_t)c * ld] : 0.0;
@@ -173,12 +177,25 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
             for (int c = 0; c < KMAX; ++c)
                 if (c >= j) rowj[buf][c] = a[c];
         }
-        // Column products reduced one column at a time.
+        // Column products (zero outside columns j..k-1): reduced together
+        // where the registers allow it (latency paid once per butterfly
+        // step), else one column at a time.
+        if constexpr (kBatchedSums) {
+            double sv[KMAX];
 #pragma unroll
-        for (int c = 0; c < KMAX; ++c) {
-            if (c >= j && c < k) {
-                const double sum = wave_sum(below ? aj * a[c] : 0.0);
-                if (lane == 0) part[buf][w][c] = sum;
+            for (int c = 0; c < KMAX; ++c) sv[c] = (below && c >= j && c < k) ? aj * a[c] : 0.0;
+            wave_sum_all<KMAX>(sv);
+            if (lane == 0) {
+#pragma unroll
+                for (int c = 0; c < KMAX; ++c) part[buf][w][c] = sv[c];
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < KMAX; ++c) {
+                if (c >= j && c < k) {
+                    const double sum = wave_sum(below ? aj * a[c] : 0.0);
+                    if (lane == 0) part[buf][w][c] = sum;
+                }
             }
         }
         __syncthreads();
@@ -229,11 +246,22 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
         const double t = taus[j];
         const double aj = pick_col(a, j);
         const double v = (i == j) ? 1.0 : ((i > j && inPanel) ? aj : 0.0);
+        if constexpr (kBatchedSums) {
+            double sv[KMAX];
 #pragma unroll
-        for (int c = 0; c < KMAX; ++c) {
-            if (c > j && c < k) {
-                const double sum = wave_sum(v * a[c]);
-                if (lane == 0) part[buf][w][c] = sum;
+            for (int c = 0; c < KMAX; ++c) sv[c] = (c > j && c < k) ? v * a[c] : 0.0;
+            wave_sum_all<KMAX>(sv);
+            if (lane == 0) {
+#pragma unroll
+                for (int c = 0; c < KMAX; ++c) part[buf][w][c] = sv[c];
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < KMAX; ++c) {
+                if (c > j && c < k) {
+                    const double sum = wave_sum(v * a[c]);
+                    if (lane == 0) part[buf][w][c] = sum;
+                }
             }
         }
         __syncthreads();
@@ -261,6 +289,22 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
 // Panel classes of a QR phase (tasks are sorted by class at upload):
 // 0: m <= 64, 1: m <= 256, 2: m <= 512 (k <= 40), 3: m <= 1024 with k <= 24
 // (16 waves of at most 128 registers fill a CU), 4: the rest (qr_kernel).
+// Latency form (batched column sums) for classes of fewer panels than this:
+// they fit the chip with room to spare, so the time is one panel's latency.
+constexpr int kQrLatencyPanels = 512;
+
+template <typename T, int KMAX, int NW>
+static void launch_qr_reg(int n, const QrTask* t, const Bases& b, hipStream_t s)
+{
+    if constexpr (KMAX == 24 && NW <= 8) {
+        if (n < kQrLatencyPanels) {
+            qr_reg_kernel<T, KMAX, NW, true><<<n, 64 * NW, 0, s>>>(b, t);
+            return;
+        }
+    }
+    qr_reg_kernel<T, KMAX, NW, false><<<n, 64 * NW, 0, s>>>(b, t);
+}
+
 template <typename T, int KMAX>
 static void launch_qr_classes(const QrTask* tasks, const int* classEnd, const Bases& b, hipStream_t s)
 {
@@ -269,10 +313,10 @@ static void launch_qr_classes(const QrTask* tasks, const int* classEnd, const Ba
         const int n = classEnd[c] - begin;
         if (n > 0) {
             const QrTask* t = tasks + begin;
-            if (c == 0) qr_reg_kernel<T, KMAX, 1><<<n, 64, 0, s>>>(b, t);
-            else if (c == 1) qr_reg_kernel<T, KMAX, 4><<<n, 256, 0, s>>>(b, t);
-            else if (c == 2) qr_reg_kernel<T, KMAX, 8><<<n, 512, 0, s>>>(b, t);
-            else qr_reg_kernel<T, 24, 16><<<n, 1024, 0, s>>>(b, t);    // class 3 has k <= 24
+            if (c == 0) launch_qr_reg<T, KMAX, 1>(n, t, b, s);
+            else if (c == 1) launch_qr_reg<T, KMAX, 4>(n, t, b, s);
+            else if (c == 2) launch_qr_reg<T, KMAX, 8>(n, t, b, s);
+            else launch_qr_reg<T, 24, 16>(n, t, b, s);    // class 3 has k <= 24
         }
         begin = classEnd[c];
     }

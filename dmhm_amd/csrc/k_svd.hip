@@ -20,6 +20,7 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
     __shared__ int order[KMAX];
     __shared__ int rotated;
     __shared__ int keep_s;
+    __shared__ double fro2;
     const SvdTask tk = tasks[blockIdx.x];
     const int ld = tk.ld > 0 ? tk.ld : tk.k;
     const int k = tk.krank >= 0 ? min(tk.k, b.rank[SLOT_WIDTH][tk.krank]) : tk.k;
@@ -29,6 +30,21 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
         const int r = e % k, c = e / k;
         As[c][r] = (double)ws[tk.roff + r + (int64_t)c * ld];
         Vs[c][r] = (r == c) ? 1.0 : 0.0;
+    }
+    // ||R||_F^2: a pair whose inner product is below 1e-30 ||R||_F^2 is taken
+    // as orthogonal. Rank-deficient factors (sketches wider than the block's
+    // rank) carry columns of rounding noise that the relative test alone
+    // would keep rotating until the sweep limit.
+    if (tid == 0) fro2 = 0.0;
+    __syncthreads();
+    {
+        double s = 0;
+        for (int e = tid; e < k * k; e += NT) {
+            const double x = As[e / k][e % k];
+            s += x * x;
+        }
+        s = wave_sum(s);
+        if ((tid & 63) == 0) atomicAdd(&fro2, s);
     }
     const int kp = k + (k & 1);
     const int pi = tid >> 3, sub = tid & 7;
@@ -53,7 +69,7 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
                     be += __shfl_xor(be, off, 64);
                     ga += __shfl_xor(ga, off, 64);
                 }
-                if (q < k && ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
+                if (q < k && fabs(ga) > 1e-15 * sqrt(al * be) && fabs(ga) > 1e-30 * fro2) {
                     const double zeta = (be - al) / (2.0 * ga);
                     const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;

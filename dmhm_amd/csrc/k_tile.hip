@@ -1,5 +1,6 @@
 // gfx950 tile GEMM engine: grouped output-owned small GEMMs (H*X, sub-H
 // products, projections, dense-leaf accumulation, factor formation).
+#include <climits>
 #include "kcommon.hpp"
 
 namespace dmhm {
@@ -84,9 +85,9 @@ __device__ __forceinline__ void tile_store(T* __restrict__ Lb, T* __restrict__ R
     }
 }
 
-template <typename T, int MR, int NC, int TM, int TN, int ROLE>
-__global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restrict__ tiles,
-                                                   const Contrib* __restrict__ cs, T alpha, T beta)
+template <typename T, int MR, int NC, int TM, int TN>
+__device__ __forceinline__ void tile_body(const Bases& b, const Tile& tl, const Contrib* __restrict__ cs, T alpha,
+                                          T beta)
 {
     constexpr int QC = TileCfg<T, MR>::QC;
     constexpr int RT = MR / TM;
@@ -96,7 +97,6 @@ __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restri
     constexpr int LP = MR + 4, RP = NC + 4;     // padded LDS rows (16-byte multiples)
     __shared__ T Ls[2][QC * LP];
     __shared__ T Rs[2][QC * RP];
-    const Tile tl = tiles[blockIdx.x];
     int m = tl.m, k = tl.k;
     if (tl.mrank >= 0) m = min(m, max(0, b.rank[tl.mslot][tl.mrank] - tl.moff));
     if (tl.krank >= 0) k = min(k, max(0, b.rank[tl.kslot][tl.krank] - tl.koff));
@@ -173,6 +173,63 @@ __global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restri
             }
         }
     }
+}
+
+// Segmented launches (merged plans of independent members): the member
+// segment holding workgroup g, found by bisection over the segments' first
+// workgroups (one or four tiles per workgroup), and the member's bases, the
+// launch's bases shifted by the member's offsets.
+template <int TPB>
+__device__ __forceinline__ int seg_of(const TileSeg* __restrict__ segs, int nseg, int g)
+{
+    if (nseg <= 64) {
+        // One load per lane and a ballot: the segment is the last one whose
+        // first workgroup is at or before g.
+        const int l = threadIdx.x & 63;
+        const int f = l < nseg ? (TPB == 1 ? segs[l].first1 : segs[l].first4) : INT_MAX;
+        return (int)__popcll(__ballot(f <= g)) - 1;
+    }
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int f = TPB == 1 ? segs[mid].first1 : segs[mid].first4;
+        if (f <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The member's bases go to LDS (tasks index them by run-time slot numbers,
+// which would put a private copy in scratch); every thread of the workgroup
+// must reach this before any wave leaves.
+template <typename T>
+__device__ __forceinline__ void seg_bases(const Bases& b, const TileSeg* __restrict__ sg, Bases& out)
+{
+    const int k = threadIdx.x;
+    if (k < NUM_SLOTS) {
+        out.val[k] = b.val[k] ? static_cast<void*>(reinterpret_cast<T*>(b.val[k]) + sg->val[k]) : nullptr;
+        out.rank[k] = b.rank[k] ? b.rank[k] + sg->rank[k] : nullptr;
+    }
+    __syncthreads();
+}
+
+template <typename T, int MR, int NC, int TM, int TN, int ROLE>
+__global__ void __launch_bounds__(256) tile_kernel(Bases b, const Tile* __restrict__ tiles,
+                                                   const Contrib* __restrict__ cs, T alpha, T beta)
+{
+    tile_body<T, MR, NC, TM, TN>(b, tiles[blockIdx.x], cs, alpha, beta);
+}
+
+template <typename T, int MR, int NC, int TM, int TN>
+__global__ void __launch_bounds__(256) tile_seg_kernel(Bases b, const TileSeg* __restrict__ segs, int nseg, T alpha,
+                                                       T beta)
+{
+    __shared__ Bases sb;
+    const TileSeg* sg = segs + seg_of<1>(segs, nseg, (int)blockIdx.x);
+    seg_bases<T>(b, sg, sb);
+    const int local = (int)blockIdx.x - sg->first1;
+    if (local >= sg->count) return;
+    tile_body<T, MR, NC, TM, TN>(sb, sg->tiles[local], sg->cs, alpha, beta);
 }
 
 // ---------------------------------------------------------------------------
@@ -290,18 +347,12 @@ __device__ __forceinline__ int mf_next(const Bases& b, const Contrib* __restrict
 }
 
 template <typename T, int MODE, int MBMAX, int NBMAX, int PF>
-__global__ void __launch_bounds__(256) tile_mfma_kernel(Bases b, const Tile* __restrict__ tiles,
-                                                        const Contrib* __restrict__ cs, int ntiles, T alpha,
-                                                        T beta)
+__device__ __forceinline__ void mfma_body(const Bases& b, const Tile& tl, const Contrib* __restrict__ cs, int rb,
+                                          T alpha, T beta)
 {
     static_assert(MODE == 0 || sizeof(T) == 4, "the BF16 mode reads FP32 operands");
     static_assert(MBMAX == 1 || MBMAX == 4, "tiles of at most 16 or 64 rows");
     typedef typename MfAcc<T>::type Acc;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int id = MBMAX == 4 ? (int)blockIdx.x : (int)blockIdx.x * 4 + w;
-    const int rb = MBMAX == 4 ? w : 0;
-    if (id >= ntiles) return;
-    const Tile tl = tiles[id];
     int m = tl.m, k = tl.k;
     if (tl.mrank >= 0) m = min(m, max(0, b.rank[tl.mslot][tl.mrank] - tl.moff));
     if (tl.krank >= 0) k = min(k, max(0, b.rank[tl.kslot][tl.krank] - tl.koff));
@@ -370,6 +421,65 @@ __global__ void __launch_bounds__(256) tile_mfma_kernel(Bases b, const Tile* __r
     }
 }
 
+template <typename T, int MODE, int MBMAX, int NBMAX, int PF>
+__global__ void __launch_bounds__(256) tile_mfma_kernel(Bases b, const Tile* __restrict__ tiles,
+                                                        const Contrib* __restrict__ cs, int ntiles, T alpha,
+                                                        T beta)
+{
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int id = MBMAX == 4 ? (int)blockIdx.x : (int)blockIdx.x * 4 + w;
+    if (id >= ntiles) return;
+    mfma_body<T, MODE, MBMAX, NBMAX, PF>(b, tiles[id], cs, MBMAX == 4 ? w : 0, alpha, beta);
+}
+
+template <typename T, int MODE, int MBMAX, int NBMAX, int PF>
+__global__ void __launch_bounds__(256) tile_mfma_seg_kernel(Bases b, const TileSeg* __restrict__ segs, int nseg,
+                                                            T alpha, T beta)
+{
+    __shared__ Bases sb;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const TileSeg* sg = segs + seg_of<MBMAX == 4 ? 1 : 4>(segs, nseg, (int)blockIdx.x);
+    seg_bases<T>(b, sg, sb);
+    const int local = MBMAX == 4 ? (int)blockIdx.x - sg->first1 : ((int)blockIdx.x - sg->first4) * 4 + w;
+    if (local >= sg->count) return;
+    mfma_body<T, MODE, MBMAX, NBMAX, PF>(sb, sg->tiles[local], sg->cs, MBMAX == 4 ? w : 0, alpha, beta);
+}
+
+// Segmented launches of a merged phase (its members' segment table).
+template <typename T, int MODE, int MBMAX, int PF>
+static void launch_mfma_seg_mb(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s)
+{
+    const int grid = MBMAX == 4 ? ph.segBlocks1 : ph.segBlocks4;
+    const int nseg = (int)ph.segSrc.size();
+    const TileSeg* sg = static_cast<const TileSeg*>(ph.dSegs);
+    const int kmax = ph.maxKTile;
+    if (grid == 0) return;
+    if (kmax <= 16) tile_mfma_seg_kernel<T, MODE, MBMAX, 1, PF><<<grid, 256, 0, s>>>(b, sg, nseg, alpha, beta);
+    else if (kmax <= 32) tile_mfma_seg_kernel<T, MODE, MBMAX, 2, PF><<<grid, 256, 0, s>>>(b, sg, nseg, alpha, beta);
+    else tile_mfma_seg_kernel<T, MODE, MBMAX, 4, PF><<<grid, 256, 0, s>>>(b, sg, nseg, alpha, beta);
+}
+
+template <typename T, int MR>
+static void launch_tile_seg_mr(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s)
+{
+    constexpr int TM = MR == 64 ? 4 : 1;
+    const int grid = ph.segBlocks1;
+    const int nseg = (int)ph.segSrc.size();
+    const TileSeg* sg = static_cast<const TileSeg*>(ph.dSegs);
+    const int kmax = ph.maxKTile;
+    if (grid == 0) return;
+    if (kmax <= 16) tile_seg_kernel<T, MR, 16, TM, 1><<<grid, 256, 0, s>>>(b, sg, nseg, alpha, beta);
+    else if (kmax <= 32) tile_seg_kernel<T, MR, 32, TM, 2><<<grid, 256, 0, s>>>(b, sg, nseg, alpha, beta);
+    else tile_seg_kernel<T, MR, 64, TM, 4><<<grid, 256, 0, s>>>(b, sg, nseg, alpha, beta);
+}
+
+template <typename T, int MODE, int PF>
+static void launch_mfma_seg(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s)
+{
+    if (ph.maxMTile <= 16) launch_mfma_seg_mb<T, MODE, 1, PF>(ph, b, alpha, beta, s);
+    else launch_mfma_seg_mb<T, MODE, 4, PF>(ph, b, alpha, beta, s);
+}
+
 template <typename T, int MODE, int MBMAX, int PF>
 static void launch_mfma_mb(int n, int kmax, const Bases& b, const Tile* t, const Contrib* c, T alpha, T beta,
                            hipStream_t s)
@@ -403,6 +513,29 @@ static void launch_tile_mr(int n, int kmax, const Bases& b, const Tile* t, const
 template <typename T>
 void launch_tile_phase(const Phase& ph, const Bases& b, T alpha, T beta, hipStream_t s, int engine)
 {
+    if (!ph.segSrc.empty()) {
+        if (ph.maxKTile > 64 || ph.maxMTile > 64) throw std::runtime_error("tile larger than 64 x 64");
+        if (!ph.dSegs) throw std::runtime_error("segmented tile phase not uploaded");
+        if (engine != TILE_LDS) {
+            const bool bf16 = (engine == TILE_MFMA_BF16 || engine == TILE_MFMA_BF16_PF) && sizeof(T) == 4;
+            const bool pf = (engine == TILE_MFMA_PF || engine == TILE_MFMA_BF16_PF) && sizeof(T) == 4;
+            if (bf16) {
+                if constexpr (sizeof(T) == 4) {
+                    if (pf) launch_mfma_seg<float, 1, 1>(ph, b, alpha, beta, s);
+                    else launch_mfma_seg<float, 1, 0>(ph, b, alpha, beta, s);
+                }
+            } else {
+                if (pf) launch_mfma_seg<T, 0, 1>(ph, b, alpha, beta, s);
+                else launch_mfma_seg<T, 0, 0>(ph, b, alpha, beta, s);
+            }
+        } else if (ph.maxMTile <= 16) {
+            launch_tile_seg_mr<T, 16>(ph, b, alpha, beta, s);
+        } else {
+            launch_tile_seg_mr<T, 64>(ph, b, alpha, beta, s);
+        }
+        HIP_LAUNCH_CHECK();
+        return;
+    }
     const int n = (int)ph.tiles.size();
     if (n == 0) return;
     const Tile* t = static_cast<const Tile*>(ph.dTiles);

@@ -72,6 +72,10 @@ void launch_permute_int(const int* src, int* dst, const int* perm, int count, hi
 struct CopyRange { int64_t src, dst, len; };
 template <typename T> void launch_copy_ranges(const T* src, T* dst, const CopyRange* ranges, int count,
                                               int64_t maxLen, bool fill, hipStream_t s);
+// Value ranges (the first nVal) and rank ranges of one allocation pair in one
+// launch; maxLen bounds every range's length.
+template <typename T> void launch_arena_ranges(T* vals, int* ranks, const CopyRange* ranges, int nVal, int nRank,
+                                               int64_t maxLen, bool fill, hipStream_t s);
 // count n x n identities at base + offs[i] (offs on the device)
 template <typename T> void launch_eye_multi(T* base, const int64_t* offs, int count, int n, hipStream_t s);
 // dst (m x m, ld m) = inverse of src (Gauss-Jordan, partial pivoting); a zero

@@ -1,5 +1,6 @@
 #include "plan.hpp"
 #include <algorithm>
+#include <climits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -521,6 +522,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     std::unordered_map<uint64_t, int64_t> t1Of;        // B block -> T1 panel
     // A 64 x 64 identity, the right operand of the gathers below.
     const int64_t eyeOff = wsAlloc(64 * 64);
+    bool eyeUsed = false;       // set by the first gather; the phase is dropped otherwise
     {
         Phase& eye = newPhase(PH_EYE);
         eye.zeroOff = eyeOff;
@@ -746,6 +748,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
                 const Box& outB = g.boxes[wf0.level][outBoxId];
                 const int64_t IN = wsAlloc((int64_t)inB.size * cap * n);
                 const int64_t OUT = wsAlloc((int64_t)outB.size * cap * n);
+                eyeUsed = true;
                 for (int i = 0; i < n; ++i) {
                     const Pending& wp = wPending[idx[i]];
                     const Block& lr = g.blocks[left ? wp.bA : wp.bB];
@@ -952,8 +955,8 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
     std::vector<Phase> kept;
     kept.reserve(plan->phases.size());
     for (auto& ph : plan->phases) {
-        const bool empty = ph.kind != PH_EYE && ph.tiles.empty() && ph.qr.empty() && ph.svd.empty() &&
-                           ph.rng.empty();
+        const bool empty = ph.kind == PH_EYE ? !eyeUsed
+                                             : ph.tiles.empty() && ph.qr.empty() && ph.svd.empty() && ph.rng.empty();
         if (!empty) kept.push_back(std::move(ph));
     }
     plan->phases.swap(kept);
@@ -970,7 +973,7 @@ Plan* Planner::hhProduct(const Layout& LA, const Layout& LB, const Layout& LC, i
 // ---------------------------------------------------------------------------
 // Merged plans
 // ---------------------------------------------------------------------------
-Plan* merge_plans(const std::vector<PlanPart>& parts)
+Plan* merge_plans(const std::vector<PlanPart>& parts, bool segmentTiles)
 {
     std::unique_ptr<Plan> out(new Plan());
     const size_t np = parts.size();
@@ -1015,6 +1018,15 @@ Plan* merge_plans(const std::vector<PlanPart>& parts)
             m.qrK = std::max(m.qrK, ph.qrK);
             switch (ph.kind) {
             case PH_TILE: {
+                if (segmentTiles) {
+                    Phase::SegSrc sg;
+                    sg.src = &ph;
+                    for (int s = 0; s < NUM_SLOTS; ++s) { sg.val[s] = vsh(i, s); sg.rank[s] = parts[i].rank[s]; }
+                    m.segSrc.push_back(sg);
+                    m.maxKTile = std::max(m.maxKTile, ph.maxKTile);
+                    m.maxMTile = std::max(m.maxMTile, ph.maxMTile);
+                    break;
+                }
                 const int cbase = (int)m.contribs.size();
                 for (Tile t : ph.tiles) {
                     t.off += vsh(i, t.slot);
@@ -1106,6 +1118,10 @@ Plan* merge_plans(const std::vector<PlanPart>& parts)
         if (ph.kind == PH_MV_ZERO) ph.zeroCount = out->wsElems;
         out->tileCount += ph.tiles.size();
         out->contribCount += ph.contribs.size();
+        for (const auto& sg : ph.segSrc) {
+            out->tileCount += sg.src->tiles.size();
+            out->contribCount += sg.src->contribs.size();
+        }
     }
     return out.release();
 }
@@ -1123,20 +1139,76 @@ std::string check_plan(const Plan& p, const int64_t* slotElems, const int64_t* r
         return off + (c - 1) * ld + r <= size(slot);
     };
     auto rankOk = [&](int slot, int idx) { return idx < 0 || (slot < NUM_SLOTS && idx < rankElems[slot]); };
-    for (size_t i = 0; i < p.phases.size(); ++i) {
-        const Phase& ph = p.phases[i];
+    const int64_t zero[NUM_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // Tiles of a phase (or of a member's source phase, with its shifts).
+    auto checkTiles = [&](size_t i, const Phase& ph, const int64_t* vs, const int64_t* rs) -> std::string {
+        auto rk = [&](int slot, int idx) { return rankOk(slot, idx < 0 ? idx : (int)(idx + rs[slot])); };
         for (const Tile& t : ph.tiles) {
-            if (!inside(t.slot, t.off, t.ld, t.m, t.k)) return where(i, "tile output out of bounds");
-            if (!rankOk(t.mslot, t.mrank) || !rankOk(t.kslot, t.krank)) return where(i, "tile rank index");
+            if (!inside(t.slot, t.off + vs[t.slot], t.ld, t.m, t.k)) return where(i, "tile output out of bounds");
+            if (!rk(t.mslot, t.mrank) || !rk(t.kslot, t.krank)) return where(i, "tile rank index");
             if (t.cb < 0 || t.ce > (int)ph.contribs.size() || t.cb > t.ce) return where(i, "tile contribution range");
             for (int q = t.cb; q < t.ce; ++q) {
                 const Contrib& c = ph.contribs[q];
                 const bool tl = c.flags & CF_TRANS_L, tr = c.flags & CF_TRANS_R;
-                const bool okL = tl ? inside(c.lslot, c.loff, c.ldl, c.q, t.m) : inside(c.lslot, c.loff, c.ldl, t.m, c.q);
-                const bool okR = tr ? inside(c.rslot, c.roff, c.ldr, t.k, c.q) : inside(c.rslot, c.roff, c.ldr, c.q, t.k);
+                const int64_t lo = c.loff + vs[c.lslot], ro = c.roff + vs[c.rslot];
+                const bool okL = tl ? inside(c.lslot, lo, c.ldl, c.q, t.m) : inside(c.lslot, lo, c.ldl, t.m, c.q);
+                const bool okR = tr ? inside(c.rslot, ro, c.ldr, t.k, c.q) : inside(c.rslot, ro, c.ldr, c.q, t.k);
                 if (!okL) return where(i, "left operand out of bounds");
                 if (!okR) return where(i, "right operand out of bounds");
-                if (!rankOk(c.qslot, c.qrank)) return where(i, "contribution rank index");
+                if (!rk(c.qslot, c.qrank)) return where(i, "contribution rank index");
+            }
+        }
+        return std::string();
+    };
+    // Footprint of a member's source phase (per slot: lowest offset, highest
+    // end, highest rank index), computed once per source and checked against
+    // every member's shifts.
+    struct Foot { int64_t lo[NUM_SLOTS], hi[NUM_SLOTS], rk[NUM_SLOTS]; bool bad; };
+    std::map<const Phase*, Foot> feet;
+    auto footprint = [&](const Phase& ph) -> const Foot& {
+        auto it = feet.find(&ph);
+        if (it != feet.end()) return it->second;
+        Foot f;
+        for (int s = 0; s < NUM_SLOTS; ++s) { f.lo[s] = INT64_MAX; f.hi[s] = -1; f.rk[s] = -1; }
+        f.bad = false;
+        auto region = [&](int slot, int64_t off, int64_t ld, int64_t r, int64_t cc) {
+            if (r <= 0 || cc <= 0) return;
+            if (slot < 0 || slot >= NUM_SLOTS || ld < r) { f.bad = true; return; }
+            f.lo[slot] = std::min(f.lo[slot], off);
+            f.hi[slot] = std::max(f.hi[slot], off + (cc - 1) * ld + r);
+        };
+        auto rank = [&](int slot, int idx) {
+            if (idx < 0) return;
+            if (slot < 0 || slot >= NUM_SLOTS) { f.bad = true; return; }
+            f.rk[slot] = std::max<int64_t>(f.rk[slot], idx);
+        };
+        for (const Tile& t : ph.tiles) {
+            region(t.slot, t.off, t.ld, t.m, t.k);
+            rank(t.mslot, t.mrank);
+            rank(t.kslot, t.krank);
+            if (t.cb < 0 || t.ce > (int)ph.contribs.size() || t.cb > t.ce) { f.bad = true; continue; }
+            for (int q = t.cb; q < t.ce; ++q) {
+                const Contrib& x = ph.contribs[q];
+                const bool tl = x.flags & CF_TRANS_L, tr = x.flags & CF_TRANS_R;
+                if (tl) region(x.lslot, x.loff, x.ldl, x.q, t.m); else region(x.lslot, x.loff, x.ldl, t.m, x.q);
+                if (tr) region(x.rslot, x.roff, x.ldr, t.k, x.q); else region(x.rslot, x.roff, x.ldr, x.q, t.k);
+                rank(x.qslot, x.qrank);
+            }
+        }
+        return feet.emplace(&ph, f).first->second;
+    };
+    for (size_t i = 0; i < p.phases.size(); ++i) {
+        const Phase& ph = p.phases[i];
+        std::string e = checkTiles(i, ph, zero, zero);
+        if (!e.empty()) return e;
+        for (const auto& sg : ph.segSrc) {
+            const Foot& f = footprint(*sg.src);
+            if (f.bad) return where(i, "member tile malformed");
+            for (int s = 0; s < NUM_SLOTS; ++s) {
+                if (f.hi[s] >= 0 && (f.lo[s] + sg.val[s] < 0 || f.hi[s] + sg.val[s] > size(s)))
+                    return where(i, "member tile operand out of bounds");
+                if (f.rk[s] >= 0 && f.rk[s] + sg.rank[s] >= rankElems[s])
+                    return where(i, "member tile rank index");
             }
         }
         for (const QrTask& t : ph.qr) {

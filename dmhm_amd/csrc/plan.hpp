@@ -19,6 +19,7 @@
 // and whole iterations can be captured in a HIP graph.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 #include "geometry.hpp"
@@ -110,6 +111,19 @@ struct MvContrib {
 
 enum PhaseKind { PH_TILE, PH_QR, PH_SVD, PH_RNG, PH_SCALE_C, PH_MV_ZERO, PH_MV_PROJ, PH_MV_ACC, PH_EYE };
 
+// One member's share of a merged tile phase that is not copied: the member
+// plan's tile and contribution arrays (device) and the shifts of its slots
+// (elements, rank indices; the workspace slot is shifted to the member's
+// workspace region). first1 / first4: the member's first workgroup when a
+// workgroup takes one / four tiles.
+struct TileSeg {
+    const Tile* tiles;
+    const Contrib* cs;
+    int64_t val[NUM_SLOTS];
+    int64_t rank[NUM_SLOTS];
+    int count, first1, first4, pad;
+};
+
 struct Phase {
     PhaseKind kind;
     std::vector<Tile> tiles;
@@ -122,6 +136,13 @@ struct Phase {
     std::vector<MvContrib> mvContribs;
     int64_t zeroOff = 0, zeroCount = 0;   // PH_MV_ZERO range in the workspace; PH_EYE: offset, order
     std::vector<int64_t> eyes;            // PH_EYE of a merged plan: every identity's workspace offset
+    // PH_TILE of a merged plan built without copying: per member, the source
+    // phase (host, uploaded with its plan) and the member's shifts; the
+    // device table of TileSeg is built at upload.
+    struct SegSrc { const Phase* src; int64_t val[NUM_SLOTS]; int64_t rank[NUM_SLOTS]; };
+    std::vector<SegSrc> segSrc;
+    void* dSegs = nullptr;
+    int segBlocks1 = 0, segBlocks4 = 0;   // workgroups of a segmented launch (1 / 4 tiles each)
     int mvK = 0, ldx = 0, ldy = 0;
     void* dAux = nullptr;
     int qrK = 0;            // panel width of the QR phase
@@ -148,6 +169,7 @@ struct Plan {
     int64_t contribCount = 0, tileCount = 0;
     int batches = 1;        // H*H: number of level-1 subtree batches sharing the workspace
     int64_t wsShared = 0;   // H*H: workspace elements shared by all batches (Omega, T1)
+    std::vector<std::shared_ptr<Plan>> members;   // member plans a segmented merged plan reads
     void upload();          // copy tile/contrib/task arrays to the device
     void release();
     ~Plan() { release(); }
@@ -209,7 +231,11 @@ struct PlanPart {
 // different parts share a launch (tasks concatenated and shifted), and every
 // part gets its own workspace region. The parts must not be uploaded (their
 // host task arrays are read) and must not write what another part reads.
-Plan* merge_plans(const std::vector<PlanPart>& parts);
+// With segmentTiles, tile phases are not copied: each merged tile phase
+// lists its members' source phases and shifts (launched through a segment
+// table); the source plans must then stay alive and be uploaded before the
+// merged plan.
+Plan* merge_plans(const std::vector<PlanPart>& parts, bool segmentTiles = false);
 
 // Host-side bounds check of a plan that has not been uploaded: every operand,
 // output, QR panel, SVD factor, sketch panel and identity of every task lies

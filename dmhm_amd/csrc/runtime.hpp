@@ -109,6 +109,8 @@ struct GeoCtx {
     // diagonal dense leaves (t == s) for AddConstantToDiagonal, per layout id
     std::map<int, std::pair<int64_t*, int*>> diagTables;
     std::map<int, int> diagCount;
+    // Layout of the transpose of a matrix of layout id (when the capacities move).
+    std::map<int, std::shared_ptr<Layout>> transLayouts;
     // Direct-inversion programs by (layout, rank, oversample).
     std::map<std::string, std::shared_ptr<DinvProgram>> dinvPrograms;
     // Device copy of the block tree (CSR import, index map), built on first use.
@@ -179,6 +181,8 @@ void direct_invert(dmhm_hmat_s* A, const dmhm_hh_opts& o);
 void direct_invert_stats(dmhm_hmat_s* A, int oversample, long* products, long* plans, double* seconds);
 void check_direct_invert_program(const Geometry& g, int maxRank, int over, long long* stats);
 int guard(const std::function<void()>& fn);
+// Frees the merged plans of batches (batch.cpp), at dmhm_finalize.
+void release_batch_plans();
 
 // Plan cache of a geometry (built and uploaded on first use) and the shared
 // workspace (grown to a plan's need).

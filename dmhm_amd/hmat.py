@@ -635,9 +635,9 @@ def check_direct_invert_program(dim, sizes, num_levels, max_rank, strong, oversa
     checks every task's bounds. Returns the program's counts."""
     sz = np.zeros(3, dtype=np.int32)
     sz[:len(sizes)] = sizes
-    st = np.zeros(5, dtype=np.int64)
+    st = np.zeros(7, dtype=np.int64)
     check(_lib.load().dmhm_check_direct_invert_program(dim, sz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), num_levels,
                                                         max_rank, 1 if strong else 0, oversample,
                                                         st.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
     return dict(products=int(st[0]), plans=int(st[1]), steps=int(st[2]), leaf_inversions=int(st[3]),
-                workspace=int(st[4]))
+                workspace=int(st[4]), phases=int(st[5]), identity_phases=int(st[6]))

@@ -160,6 +160,21 @@ def config_operator(name, seed=0):
                 colind=colind, vals=vals, map=hmap)
 
 
+def c3_batch(count=48, seed=0, sides=(16, 20, 24, 28, 32), num_levels=4, max_rank=32):
+    """The batch of independent 3D operators: -div(a grad u) + u on n^3 grids,
+    n drawn uniformly from `sides`, a = 1 + 0.5 U[0,1) i.i.d. per cell, each
+    with its own seed; HMat3d numLevels 4, maxRank 32."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.asarray(sides), size=count)
+    out = []
+    for i, n in enumerate(ns):
+        a = coefficient_uniform_3d(int(n), seed * 1000 + i)
+        rowptr, colind, vals, sizes, hmap = flux_operator(a, num_levels)
+        out.append(dict(dim=3, sizes=sizes, num_levels=num_levels, max_rank=max_rank, rowptr=rowptr,
+                        colind=colind, vals=vals, map=hmap))
+    return out
+
+
 def small_operator(n, num_levels, dim=2, smooth=True, seed=0):
     """A small operator of the same family for tests (n per axis)."""
     if dim == 2:

@@ -288,9 +288,9 @@ int dmhm_hmat_direct_invert(dmhm_hmat h, const dmhm_hh_opts* opts);
  * seconds it took to build (builds it if needed). No reference counterpart. */
 /* Host only (no GPU needed): lowers the direct inversion of a geometry at
  * rank maxRank and checks that every task of the program stays inside its
- * buffers. stats[5]: sub-matrix products, merged plans, steps, dense leaf
- * inversions, workspace elements. DMHM_EINVAL names the first violation. No
- * reference counterpart. */
+ * buffers. stats[7]: sub-matrix products, merged plans, steps, dense leaf
+ * inversions, workspace elements, plan phases (launches), identity phases.
+ * DMHM_EINVAL names the first violation. No reference counterpart. */
 int dmhm_check_direct_invert_program(int dim, const int* sizes, int numLevels, int maxRank, int strong,
                                      int oversample, long long* stats);
 int dmhm_direct_invert_program_stats(dmhm_hmat h, int oversample, long long* products, long long* plans,
@@ -328,6 +328,10 @@ int dmhm_hbatch_schulz_invert(dmhm_hbatch b, const dmhm_schulz_opts* opts, dmhm_
 int dmhm_hmat_packed_size(dmhm_hmat h, uint64_t* bytes);
 int dmhm_hmat_pack(dmhm_hmat h, void* buf, uint64_t capacity, uint64_t* written);
 int dmhm_hmat_unpack(const void* buf, uint64_t size, dmhm_hmat* out);
+/* The same from an image of implicit length (HMat2d::Unpack(const byte*),
+ * include/dmhm/hmat2d.hpp:111): the length follows from the image's header
+ * and rank table; *consumed (if not NULL) receives it. */
+int dmhm_hmat_unpack_image(const void* buf, uint64_t* consumed, dmhm_hmat* out);
 
 /* Expand to a dense n x n column-major host matrix (HMat2d::Print multiplies
  * by the identity, HMat2d.cpp:233-248). For tests on small matrices. */

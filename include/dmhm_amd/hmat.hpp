@@ -324,6 +324,14 @@ public:
         replace([&](dmhm_hmat* out) { return dmhm_hmat_unpack(packedHMat, size, out); });
         return size;
     }
+    // The reference's form (include/dmhm/hmat2d.hpp:111): the image carries
+    // its own length; returns the bytes read.
+    std::size_t Unpack(const byte* packedHMat)
+    {
+        uint64_t n = 0;
+        replace([&](dmhm_hmat* out) { return dmhm_hmat_unpack_image(packedHMat, &n, out); });
+        return (std::size_t)n;
+    }
     std::size_t Unpack(const Vector<byte>& packedHMat)
     {
         return Unpack(packedHMat.LockedBuffer(), (std::size_t)packedHMat.Size());

@@ -241,6 +241,14 @@ static int run_gpu()
     bool same = image.Size() == image2.Size();
     for (int i = 0; same && i < image.Size(); ++i) same = image[i] == image2[i];
     expect(same && (std::size_t)image.Size() == X64.PackedSize(), "Pack / Unpack round trip", image.Size());
+    // Unpack(const byte*): the image's own length.
+    HMat2d<double> Xu;
+    const std::size_t read = Xu.Unpack(image.LockedBuffer());
+    Vector<byte> image3;
+    Xu.Pack(image3);
+    same = read == (std::size_t)image.Size() && image3.Size() == image.Size();
+    for (int i = 0; same && i < image.Size(); ++i) same = image[i] == image3[i];
+    expect(same, "Unpack(const byte*) reads the image's own length", (double)read);
 
     // Error mapping: nonconformal H*H raises std::logic_error.
     bool threw = false;
