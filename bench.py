@@ -7,10 +7,12 @@ correlation length 0.1, seed 0), HMat2d numLevels = 6, maxRank = 16, strong
 admissibility, to an estimate of ||I - X A||_2 below 1e-8 or the rank floor
 (the estimate stops decreasing). The timed form starts the Schulz iteration
 X_{k+1} = (2I - X_k A) X_k from the recursive direct inversion
-(HMat2d::DirectInvert) and refines in FP64. Timed once beside it: the direct
-inversion alone, the mixed form from the direct start, and the reference's own
-start X_0 = alpha A^T in plain FP64 and mixed (FP32 iterations until the
-estimate is below 1e-3). Also reported: H*X on 1 and 64 FP32 right-hand sides
+(HMat2d::DirectInvert) and refines in FP64 (from that start every step runs in
+FP64: it already sits at the truncation floor). Timed once beside it: the
+direct inversion alone, and the reference's own start X_0 = alpha A^T in plain
+FP64 and mixed (FP32 iterations until the estimate is below 1e-3, then FP64),
+the latter also with BF16 tile operands (FP32 accumulation) while the
+estimate is above 0.5. Also reported: H*X on 1 and 64 FP32 right-hand sides
 (HBM GB/s, roofline of the accumulation launch), one steady-state H*H product
 (TFLOP/s), and the FP64 CPU oracle on the host cores.
 
@@ -138,6 +140,44 @@ def cpu_baseline(sample, iterations):
             "per_iteration_seconds": per_iter, "iterations": iterations, "samples": sample["samples"]}
 
 
+def batch_mode(a, np, dmhm_amd, W):
+    """The batch of independent 3D operators (-div(a grad u) + u, a = 1 + 0.5
+    U[0,1) per cell, HMat3d numLevels 4, maxRank 32, FP32; sides drawn from
+    --batch-sides): --batch-iters Schulz steps from X_0 = alpha A^T, all
+    members in one launch per phase, then the same steps member by member.
+    Each form runs once untimed (plans, workspace, tile-engine tuning) and
+    once timed."""
+    sync = dmhm_amd.load().dmhm_synchronize
+    sides = tuple(int(s) for s in a.batch_sides.split(","))
+    log("bench: building %d 3D operators (sides %s)" % (a.batch_count, sides))
+    ops = W.c3_batch(a.batch_count, sides=sides)
+    mats = [dmhm_amd.HMat3d.from_csr(o["sizes"], 4, 32, True, o["rowptr"], o["colind"], o["vals"],
+                                     precision=dmhm_amd.DMHM_F32) for o in ops]
+    K = a.batch_iters
+    out = {"mode": "batch", "members": len(mats), "sides": [int(o["sizes"][0]) for o in ops],
+           "unknowns": int(sum(np.prod(o["sizes"]) for o in ops)), "schulz_steps": K, "precision": "f32"}
+    for tag in ("untimed", "timed"):
+        B = dmhm_amd.HBatch.create(mats)
+        sync()
+        t = time.perf_counter()
+        B.SchulzInvert(numIterations=K)
+        sync()
+        out["batched_%s_s" % tag] = time.perf_counter() - t
+        log("bench: batched %s %.3f s" % (tag, out["batched_%s_s" % tag]))
+        del B
+    for tag in ("untimed", "timed"):
+        sync()
+        t = time.perf_counter()
+        for m in mats:
+            c = m.Copy()
+            c.SchulzInvert(numIterations=K)
+        sync()
+        out["sequential_%s_s" % tag] = time.perf_counter() - t
+        log("bench: one by one %s %.3f s" % (tag, out["sequential_%s_s" % tag]))
+    out["batched_speedup"] = out["sequential_timed_s"] / out["batched_timed_s"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,9 +188,13 @@ def main():
     ap.add_argument("--switch-tol", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--mode", choices=["full", "matvec", "hh", "inverse"], default="full",
+    ap.add_argument("--mode", choices=["full", "matvec", "hh", "inverse", "batch"], default="full",
                     help="matvec / hh / inverse: only that loop (for rocprofv3 runs); inverse: the timed "
-                         "inversion of the full mode, --steps times")
+                         "inversion of the full mode, --steps times; batch: a batch of independent 3D "
+                         "operators inverted together and one by one")
+    ap.add_argument("--batch-count", type=int, default=48)
+    ap.add_argument("--batch-sides", default="16,20,24,28,32")
+    ap.add_argument("--batch-iters", type=int, default=2, help="Schulz steps of the batch mode")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--config", choices=["c2", "c4"], default="c2",
                     help="c2: 2D 256^2 (the headline workload); c4: 2D 512^2, sigma=1.5, l=0.05, "
@@ -166,6 +210,9 @@ def main():
     from dmhm_amd import workloads as W
 
     dmhm_amd.init(0)
+    if a.mode == "batch":
+        batch_mode(a, np, dmhm_amd, W)
+        return
     if a.config == "c4" and a.mode == "full":
         print("the 512^2 workload is measured in the kernel modes (--mode matvec / hh)", file=sys.stderr)
         sys.exit(2)
@@ -227,11 +274,9 @@ def main():
         X = A.Copy()
         X.SchulzInvert(numIterations=2, start="direct")
         if w == 0:
-            for mixed in (False, True):
+            for mixed, bf16 in ((False, False), (True, False), (True, True)):
                 X = A.Copy()
-                X.SchulzInvert(numIterations=2, mixed=mixed)
-            X = A.Copy()
-            X.SchulzInvert(numIterations=2, mixed=True, start="direct")
+                X.SchulzInvert(numIterations=2, mixed=mixed, bf16=bf16)
     sync()
 
     # The timed step: the inverse to ||I - X A||_2 < stopTol (or the rank
@@ -267,13 +312,9 @@ def main():
     sec_direct = time.perf_counter() - t
     res_direct = residual_power_estimate(A, Xd, n)
     log("bench: direct inversion alone %.3f s, power check %.3e" % (sec_direct, res_direct))
-    # The mixed form from the direct start (X_0 from an FP32 direct inversion).
-    Xm, repm, sec_mixed = timed(start="direct", mixed=True, **sopts)
-    res_mixed = residual_power_estimate(A, Xm, n)
-    log("bench: direct start mixed %.3f s, %d iterations (%d FP32), estimate %.3e, power check %.3e"
-        % (sec_mixed, repm.iterations, repm.iterationsLowPrecision, repm.residualEstimate, res_mixed))
     # The reference's own start X_0 = alpha A^T, plain FP64 and mixed.
     sec64 = rep64 = res64 = sec_ref_mixed = rep_ref_mixed = res_ref_mixed = None
+    sec_ref_bf16 = rep_ref_bf16 = res_ref_bf16 = None
     if not a.no_fp64:
         X64, rep64, sec64 = timed(mixed=False, **sopts)
         res64 = residual_power_estimate(A, X64, n)
@@ -284,6 +325,11 @@ def main():
         log("bench: reference start mixed %.2f s, %d iterations (%d FP32), estimate %.3e, power check %.3e"
             % (sec_ref_mixed, rep_ref_mixed.iterations, rep_ref_mixed.iterationsLowPrecision,
                rep_ref_mixed.residualEstimate, res_ref_mixed))
+        Xb, rep_ref_bf16, sec_ref_bf16 = timed(mixed=True, bf16=True, **sopts)
+        res_ref_bf16 = residual_power_estimate(A, Xb, n)
+        log("bench: reference start mixed+BF16 %.2f s, %d iterations (%d FP32, %d BF16), estimate %.3e, "
+            "power check %.3e" % (sec_ref_bf16, rep_ref_bf16.iterations, rep_ref_bf16.iterationsLowPrecision,
+                                  rep_ref_bf16.iterationsBf16, rep_ref_bf16.residualEstimate, res_ref_bf16))
         v = np.random.default_rng(9).standard_normal(n)
         diff = np.linalg.norm(X.Multiply(1.0, v) - X64.Multiply(1.0, v)) / np.linalg.norm(X.Multiply(1.0, v))
     else:
@@ -348,10 +394,6 @@ def main():
                    "reached_1e-8_power_check": bool(res < 1e-8),
                    "direct_invert_seconds": sec_direct, "direct_invert_residual_power_check": res_direct,
                    "direct_invert_products": dstats[0], "direct_invert_merged_plans": dstats[1],
-                   "direct_start_mixed_seconds": sec_mixed, "direct_start_mixed_iterations": repm.iterations,
-                   "direct_start_mixed_fp32_iterations": repm.iterationsLowPrecision,
-                   "direct_start_mixed_residual_estimate": repm.residualEstimate,
-                   "direct_start_mixed_residual_power_check": res_mixed,
                    "reference_start_fp64_seconds": sec64,
                    "reference_start_fp64_iterations": rep64.iterations if rep64 else None,
                    "reference_start_fp64_residual_estimate": rep64.residualEstimate if rep64 else None,
@@ -361,6 +403,13 @@ def main():
                    "reference_start_mixed_residual_estimate":
                        rep_ref_mixed.residualEstimate if rep_ref_mixed else None,
                    "reference_start_mixed_residual_power_check": res_ref_mixed,
+                   "reference_start_mixed_bf16_seconds": sec_ref_bf16,
+                   "reference_start_mixed_bf16_iterations": rep_ref_bf16.iterations if rep_ref_bf16 else None,
+                   "reference_start_mixed_bf16_bf16_iterations":
+                       rep_ref_bf16.iterationsBf16 if rep_ref_bf16 else None,
+                   "reference_start_mixed_bf16_residual_estimate":
+                       rep_ref_bf16.residualEstimate if rep_ref_bf16 else None,
+                   "reference_start_mixed_bf16_residual_power_check": res_ref_bf16,
                    "speedup_vs_reference_start_fp64": (sec64 / sec) if sec64 else None,
                    "direct_vs_reference_fp64_relative_difference": diff},
         "matvec": {"f32_k1_ms": ms1[0], "f32_k1_GBps": gbs1, "f32_k1_accumulation_GBps": acc_gbs,

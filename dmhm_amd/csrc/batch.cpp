@@ -139,7 +139,10 @@ static Plan* merged_product(const Arena& A, const Arena& B, const Arena& C, int 
         if (k == cls.size()) cls.push_back({a, b, c, nullptr});
         of[m] = k;
     }
-    const int64_t budget = std::max<int64_t>(hh_batch_budget() / (int64_t)M, int64_t(1) << 24);
+    // Each class plan is the plan of a lone member's product (same workspace
+    // budget, so the same level-1 batches and phases): members merge phase
+    // for phase, and a batch launches what one member launches.
+    const int64_t budget = hh_batch_budget();
     build_parallel(cls.size(), [&](size_t k) {
         Cls& x = cls[k];
         Planner pl(x.a->geo());
@@ -147,28 +150,48 @@ static Plan* merged_product(const Arena& A, const Arena& B, const Arena& C, int 
         pl.setStacking(x.a->full, x.b->full);
         x.plan.reset(pl.hhProduct(*x.a->lay, *x.b->lay, *x.c->lay, x.c->maxRank, oversample));
     });
-    std::vector<PlanPart> parts(M);
-    for (size_t m = 0; m < M; ++m) {
-        PlanPart& p = parts[m];
-        p.plan = cls[of[m]].plan.get();
-        p.val[SLOT_A] = A.val[m]; p.rank[SLOT_A] = A.rank[m];
-        p.val[SLOT_B] = B.val[m]; p.rank[SLOT_B] = B.rank[m];
-        p.val[SLOT_C] = C.val[m]; p.rank[SLOT_C] = C.rank[m];
-    }
-    // Tile phases by segment: members of a class share their class plan's
-    // tile arrays on the device (no copies); the smaller task lists (QR,
-    // SVD, sketches) are concatenated.
-    std::unique_ptr<Plan> merged(merge_plans(parts, true));
+    std::unique_ptr<Plan> holder(new Plan());
+    // Chunks of consecutive members whose workspaces together fit the budget
+    // (at least one member each), one merged plan per chunk; the chunks run
+    // one after another in one shared workspace.
     const int64_t slots[NUM_SLOTS] = {A.totalVals, B.totalVals, C.totalVals, 0, 0, 0, 0, 0};
     const int64_t ranks[NUM_SLOTS] = {A.totalRanks, B.totalRanks, C.totalRanks, 0, 0, 0, 0, 0};
-    const std::string e = check_plan(*merged, slots, ranks);
-    if (!e.empty()) throw Error(DMHM_EINVAL, "merged batch product: " + e);
+    for (size_t m0 = 0; m0 < M;) {
+        int64_t ws = 0;
+        size_t m1 = m0;
+        while (m1 < M) {
+            const int64_t w = (cls[of[m1]].plan->wsElems + 63) & ~int64_t(63);
+            if (m1 > m0 && ws + w > budget) break;
+            ws += w;
+            ++m1;
+        }
+        std::vector<PlanPart> parts(m1 - m0);
+        for (size_t m = m0; m < m1; ++m) {
+            PlanPart& p = parts[m - m0];
+            p.plan = cls[of[m]].plan.get();
+            p.val[SLOT_A] = A.val[m]; p.rank[SLOT_A] = A.rank[m];
+            p.val[SLOT_B] = B.val[m]; p.rank[SLOT_B] = B.rank[m];
+            p.val[SLOT_C] = C.val[m]; p.rank[SLOT_C] = C.rank[m];
+        }
+        // Tile phases by segment (no copies of the class plans' tiles); the
+        // smaller task lists (QR, SVD, sketches) are concatenated.
+        std::unique_ptr<Plan> merged(merge_plans(parts, true));
+        const std::string e = check_plan(*merged, slots, ranks);
+        if (!e.empty()) throw Error(DMHM_EINVAL, "merged batch product: " + e);
+        holder->wsElems = std::max(holder->wsElems, merged->wsElems);
+        holder->flops += merged->flops;
+        holder->chunks.push_back(std::move(merged));
+        m0 = m1;
+    }
+    // The checks above read the class plans' host arrays, which an upload
+    // releases; segmented tile phases then read the class plans' arrays on
+    // the device, so the class plans go up first.
     for (auto& x : cls) {
         x.plan->upload();
-        merged->members.push_back(x.plan);
+        holder->members.push_back(x.plan);
     }
-    merged->upload();
-    return merged.release();
+    for (auto& c : holder->chunks) c->upload();
+    return holder.release();
 }
 
 // Merged streaming H*x (one right-hand side) of every member; member i's x and
@@ -362,8 +385,10 @@ static void batch_multiply_into(dmhm_hbatch_s& P, double alpha, const Arena& A, 
     b.val[SLOT_C] = C.vals.ptr; b.rank[SLOT_C] = static_cast<int*>(C.ranks.ptr);
     b.val[SLOT_WS] = g_lib.ws.ptr;
     const TruncPolicy pol{o.tolMode, o.relTol};
-    if (A.prec == DMHM_F64) run_plan<double>(p, b, alpha, 0.0, pol, o.seed);
-    else run_plan<float>(p, b, (float)alpha, 0.0f, pol, o.seed);
+    for (auto& chunk : p.chunks) {
+        if (A.prec == DMHM_F64) run_plan<double>(*chunk, b, alpha, 0.0, pol, o.seed);
+        else run_plan<float>(*chunk, b, (float)alpha, 0.0f, pol, o.seed);
+    }
     for (size_t i = 0; i < C.size(); ++i) C.at(i)->full = true;
 }
 
@@ -500,6 +525,7 @@ static void batch_schulz(dmhm_hbatch_s& B, const dmhm_schulz_opts& o, dmhm_schul
             if (steady && g_lib.untuned == u0) tuned = true;
         }
         ++steps;
+        const Arena* Xprev = Xc;
         Xc = &X[side];
         side ^= 1;
         bool all = true;
@@ -508,11 +534,12 @@ static void batch_schulz(dmhm_hbatch_s& B, const dmhm_schulz_opts& o, dmhm_schul
             for (size_t m = 0; m < M; ++m) {
                 if (doneAt[m]) continue;
                 hist[m].push_back(est[m]);
-                const size_t n = hist[m].size();
-                const bool stalled = o.stallStop && n >= 3 && hist[m][n - 1] < 0.1 && hist[m][n - 1] > 0.5 * hist[m][n - 3];
+                const bool stalled = o.stallStop && stall_detected(hist[m]);
                 if (std::fabs(est[m]) < o.stopTol || stalled) {
+                    // At the floor the member keeps the step's input X_k, as
+                    // the single-operator iteration does.
                     doneAt[m] = it;
-                    copy_member(R.at(m), Xc->at(m));
+                    copy_member(R.at(m), (stalled && !(std::fabs(est[m]) < o.stopTol) ? Xprev : Xc)->at(m));
                 } else {
                     all = false;
                 }

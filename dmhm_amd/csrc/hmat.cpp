@@ -921,28 +921,48 @@ double estimate_two_norm(dmhm_hmat_s* h, double theta, double confidence, uint64
 
 
 // One Schulz step X_out = (2I - X_in A) X_in (Invert-incl.hpp:160-192) as
-// launches only: the counter bump, Z = -X_in A, Z += I, the residual power
-// iteration on Z (stop mode), Z += I, X_out = Z X_in. Every seed is the
-// step's fixed base plus the device counter, so a captured step replayed by a
-// HIP graph launches exactly what the eager step launches.
+// launches only, in two halves. head(): the counter bump, Z = -X_in A, Z += I
+// and, in stop mode, the residual power iteration on Z; tail(): Z += I,
+// X_out = Z X_in. The stop decision reads the estimate between the halves, so
+// a step whose estimate shows the iteration at its floor skips the second
+// product. Every seed is the step's fixed base plus the device counter, so a
+// captured half replayed by a HIP graph launches exactly what the eager half
+// launches.
 struct SchulzStep {
     dmhm_hmat_s *A, *Xin, *Xout, *Z;
     bool stop;
     double theta, confidence;
     dmhm_hh_opts hh;
     uint64_t seed;
-    void launch() const {
+    void head() const {
         launch_bump(g_lib.dCounter, g_lib.stream);
         dmhm_hh_opts o = hh;
         o.seed = seed;
         multiply_into(-1.0, Xin, A, Z, o);
         add_diag(Z, 1.0);
         if (stop) launch_two_norm(Z, theta, confidence, seed + 1);
+    }
+    void tail() const {
         add_diag(Z, 1.0);
+        dmhm_hh_opts o = hh;
         o.seed = seed + 2;
         multiply_into(1.0, Z, Xin, Xout, o);
     }
 };
+
+// The truncation floor of a Schulz iteration: the estimate is below 0.1 and
+// either barely moved in the last step (above 0.9 of the previous one) or did
+// not halve over the last two. Below 0.1 an iteration in its quadratic regime
+// shrinks the estimate to at most theta times its square per step, and one
+// whose products lose accuracy to truncation still contracts by more than
+// half over two steps while it has room above the floor.
+bool stall_detected(const std::vector<double>& h)
+{
+    const size_t n = h.size();
+    if (n < 2 || !(h[n - 1] < 0.1)) return false;
+    if (h[n - 1] > 0.9 * h[n - 2]) return true;
+    return n >= 3 && h[n - 1] > 0.5 * h[n - 3];
+}
 
 // SchulzInvert (Invert-incl.hpp:128-196): X_0 = alpha A^T, alpha = 2/est^2,
 // X <- (2I - X A) X. The iterates live in two preallocated buffers of the
@@ -968,11 +988,14 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
     struct PrecState {
         dmhm_hmat_s* A = nullptr;
         std::unique_ptr<dmhm_hmat_s> Aown, X[2], Z;
-        StepGraph g[2][2];                  // [BF16 operands][ping-pong direction]
-        bool captured[2][2] = {{false, false}, {false, false}};
+        StepGraph g[2][2][2];               // [BF16 operands][ping-pong direction][half]
+        bool captured[2][2][2] = {};
         bool tuned[2] = {false, false};     // an eager steady step ran with every phase tuned
     } ps[2];
-    int cur = (o.mixed && A->prec == DMHM_F64) ? 1 : (A->prec == DMHM_F64 ? 0 : 1);
+    // The direct start already sits at the truncation floor of A's precision,
+    // where lower-precision steps could only add rounding error: it runs every
+    // step in A's precision.
+    int cur = (o.mixed && o.start != 1 && A->prec == DMHM_F64) ? 1 : (A->prec == DMHM_F64 ? 0 : 1);
     const int target = A->prec == DMHM_F64 ? 0 : 1;    // precision of the result
     auto prepare = [&](int pi) {
         PrecState& P = ps[pi];
@@ -1001,12 +1024,10 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
     }
 
     int it = 0, itLow = 0, itGraph = 0, itBf16 = 0;
+    bool bf16Off = false;
     double e = -1;
     std::vector<double> hist;
-    auto stalled = [&]() {
-        const size_t n = hist.size();
-        return o.stallStop && n >= 3 && hist[n - 1] < 0.1 && hist[n - 1] > 0.5 * hist[n - 3];
-    };
+    auto stalled = [&]() { return o.stallStop && stall_detected(hist); };
     dmhm_hmat_s* Xc = X0.get();       // current iterate
     int side = 0;                     // X[side] receives the next iterate
     int stepsInPrec = 0;
@@ -1020,8 +1041,10 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
         if (low) ++itLow;
         PrecState& P = ps[cur];
         // BF16 operands (FP32 accumulation) for the early low-precision steps:
-        // until the residual estimate drops below bf16Tol.
-        const int bf = (low && cur == 1 && o.bf16 && (hist.empty() || std::fabs(e) >= o.bf16Tol)) ? 1 : 0;
+        // until the residual estimate drops below bf16Tol, or for good once an
+        // estimate grew by more than 2% over the previous one (near a
+        // residual of 1 the contraction is too weak to absorb BF16 rounding).
+        const int bf = (low && cur == 1 && o.bf16 && !bf16Off && (hist.empty() || std::fabs(e) >= o.bf16Tol)) ? 1 : 0;
         if (bf) ++itBf16;
         Bf16Scope bscope(bf != 0);
         SchulzStep st{P.A, Xc, P.X[side].get(), P.Z.get(), stop, o.theta, o.confidence, o.hh,
@@ -1029,29 +1052,49 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
         // Eager steps build the plans, size the workspace and autotune the
         // tile engines (the very first step reads X_0, whose layout and rank
         // profile differ); once a steady step of this precision and operand
-        // mode ran with every phase tuned, later steps replay captured graphs.
+        // mode ran with every phase tuned, later steps replay captured graphs
+        // (one per half and ping-pong direction).
         const bool steady = Xc == P.X[0].get() || Xc == P.X[1].get();
-        if (o.graph && steady && stepsInPrec >= 2 && P.tuned[bf]) {
-            if (!P.captured[bf][side]) { P.g[bf][side].capture([&] { st.launch(); }); P.captured[bf][side] = true; }
-            P.g[bf][side].replay();
-            ++itGraph;
-        } else {
-            const long u0 = g_lib.untuned;
-            st.launch();
-            if (steady && g_lib.untuned == u0) P.tuned[bf] = true;
-        }
+        const bool replay = o.graph && steady && stepsInPrec >= 2 && P.tuned[bf];
+        const long u0 = g_lib.untuned;
+        auto half = [&](int h) {
+            if (!replay) { h == 0 ? st.head() : st.tail(); return; }
+            StepGraph& G = P.g[bf][side][h];
+            if (!P.captured[bf][side][h]) {
+                G.capture([&] { h == 0 ? st.head() : st.tail(); });
+                P.captured[bf][side][h] = true;
+            }
+            G.replay();
+        };
+        half(0);
+        if (replay) ++itGraph;
         ++stepsInPrec;
-        Xc = P.X[side].get();
-        side ^= 1;
-        bool done = false;
+        bool done = false, skipTail = false;
         if (stop) {
             e = read_two_norm(o.theta);
             if (rep && kk < 128) rep->history[kk] = e;
             hist.push_back(e);
-            if (!low && (std::fabs(e) < o.stopTol || stalled())) done = true;
+            if (bf && hist.size() >= 2 && e > 1.02 * hist[hist.size() - 2]) bf16Off = true;
+            const bool atFloor = stalled();
+            // At the floor the second product cannot improve X: keep X_k.
+            if (!low && std::fabs(e) < o.stopTol) done = true;          // the reference's stop (with X_k+1)
+            else if (!low && atFloor) { done = true; skipTail = true; }
+            else if (low && atFloor) skipTail = true;
+        }
+        if (!skipTail) {
+            if (bf && bf16Off) {
+                // The step whose estimate grew finishes with FP32 operands.
+                g_lib.bf16 = false;
+                st.tail();
+            } else {
+                half(1);
+            }
+            if (!replay && steady && g_lib.untuned == u0) P.tuned[bf] = true;
+            Xc = P.X[side].get();
+            side ^= 1;
         }
         if (done) break;
-        if (low && stop && (std::fabs(e) < o.switchTol || stalled())) {
+        if (low && stop && (std::fabs(e) < o.switchTol || skipTail)) {
             // Finish in the result's precision: convert the iterate into its buffers.
             prepare(target);
             PrecState& T = ps[target];

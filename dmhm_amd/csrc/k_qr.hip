@@ -286,6 +286,111 @@ __global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* 
     }
 }
 
+// Householder QR of a panel of m <= NW*RW rows and k <= 64 columns by NW
+// waves with the panel in registers, transposed to the other kernels: lane c
+// holds column c, wave w rows RW*w .. RW*w+RW-1. Step j: each lane reads the
+// reflector column j from lane j of its wave (readlane, uniform) and forms
+// its column's product with it over the wave's rows; the NW wave partials
+// (column j's own product is the squared norm below the diagonal) are summed
+// through LDS, where the wave holding row j also publishes that row. One
+// barrier per step (the LDS buffers alternate) and no cross-lane reduction.
+// Q is formed in place in orgqr order the same way. Same reflector
+// convention, R and Q as qr_kernel.
+template <typename T, int RW, int NW>
+__global__ void __launch_bounds__(64 * NW) qr_col_kernel(Bases b, const QrTask* __restrict__ tasks)
+{
+    __shared__ double part[2][NW][64];
+    __shared__ double rowj[2][64];
+    __shared__ double taus[64];
+    const QrTask tk = tasks[blockIdx.x];
+    T* A = reinterpret_cast<T*>(b.val[tk.slot]) + tk.off;
+    const int m = tk.m, kstat = tk.k, ld = tk.ld;
+    const int k = tk.krank >= 0 ? min(kstat, b.rank[tk.kslot][tk.krank]) : kstat, kk = min(m, k);
+    if (k <= 0) return;
+    const int c = threadIdx.x & 63, w = threadIdx.x >> 6, r0 = w * RW;
+    double a[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) a[r] = (c < k && r0 + r < m) ? (double)A[r0 + r + (int64_t)c * ld] : 0.0;
+#pragma unroll 1
+    for (int j = 0; j < kk; ++j) {
+        const int buf = j & 1;
+        double dot = 0.0;
+#pragma unroll
+        for (int r = 0; r < RW; ++r)
+            if (r0 + r > j) dot += bcast(a[r], j) * a[r];
+        part[buf][w][c] = dot;
+        if (w == j / RW) {
+            double v = 0.0;
+#pragma unroll
+            for (int r = 0; r < RW; ++r) v = (r0 + r == j) ? a[r] : v;
+            rowj[buf][c] = v;
+        }
+        __syncthreads();
+        double xn2 = 0.0, dc = 0.0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) { xn2 += part[buf][q][j]; dc += part[buf][q][c]; }
+        const double alpha = rowj[buf][j];
+        double t = 0.0, scale = 0.0, beta = alpha;
+        if (xn2 > 0.0 && alpha * alpha + xn2 > kTinyColumn) {
+            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+            t = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        const double wc = t * (rowj[buf][c] + scale * dc);
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int i = r0 + r;
+            const double vi = bcast(a[r], j) * scale;     // reflector entry of row i (rows below j)
+            if (c == j) a[r] = (i == j) ? beta : ((i > j) ? vi : a[r]);
+            else if (c > j && i == j) a[r] -= wc;
+            else if (c > j && i > j) a[r] -= wc * vi;
+        }
+        if (threadIdx.x == 0) taus[j] = t;
+    }
+    __syncthreads();
+    if (tk.roff >= 0 && c < k) {
+        T* R = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.roff;
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int i = r0 + r;
+            if (i < kstat) R[i + (int64_t)c * tk.rld] = (i <= c && i < kk) ? (T)a[r] : T(0);
+        }
+    }
+    if (c >= kk) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r) a[r] = 0.0;
+    }
+#pragma unroll 1
+    for (int j = kk - 1; j >= 0; --j) {
+        const int buf = j & 1;
+        const double t = taus[j];
+        double dot = 0.0;
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int i = r0 + r;
+            const double v = (i == j) ? 1.0 : ((i > j) ? bcast(a[r], j) : 0.0);
+            dot += v * a[r];
+        }
+        part[buf][w][c] = dot;
+        __syncthreads();
+        double dc = 0.0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) dc += part[buf][q][c];
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int i = r0 + r;
+            const double v = (i == j) ? 1.0 : ((i > j) ? bcast(a[r], j) : 0.0);
+            if (c == j) a[r] = (i == j) ? 1.0 - t : ((i > j) ? -t * v : 0.0);
+            else if (c > j && c < k) a[r] -= t * dc * v;
+        }
+    }
+    if (c < k) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r)
+            if (r0 + r < m) A[r0 + r + (int64_t)c * ld] = (T)a[r];
+    }
+}
+
 // Panel classes of a QR phase (tasks are sorted by class at upload):
 // 0: m <= 64, 1: m <= 256, 2: m <= 512 (k <= 40), 3: m <= 1024 with k <= 24
 // (16 waves of at most 128 registers fill a CU), 4: the rest (qr_kernel).
@@ -313,8 +418,17 @@ static void launch_qr_classes(const QrTask* tasks, const int* classEnd, const Ba
         const int n = classEnd[c] - begin;
         if (n > 0) {
             const QrTask* t = tasks + begin;
-            if (c == 0) launch_qr_reg<T, KMAX, 1>(n, t, b, s);
-            else if (c == 1) launch_qr_reg<T, KMAX, 4>(n, t, b, s);
+            // Few panels (latency): the column-per-lane kernel with more waves
+            // than rows per wave; many panels (throughput): the row-per-lane
+            // kernel, except for wide panels of <= 64 rows.
+            const bool few = n < kQrLatencyPanels;
+            if (c == 0) {
+                if (few || KMAX > 24) qr_col_kernel<T, 8, 8><<<n, 512, 0, s>>>(b, t);
+                else qr_reg_kernel<T, KMAX, 1, false><<<n, 64, 0, s>>>(b, t);
+            } else if (c == 1) {
+                if (few) qr_col_kernel<T, 16, 16><<<n, 1024, 0, s>>>(b, t);
+                else qr_reg_kernel<T, KMAX, 4, false><<<n, 256, 0, s>>>(b, t);
+            }
             else if (c == 2) launch_qr_reg<T, KMAX, 8>(n, t, b, s);
             else launch_qr_reg<T, 24, 16>(n, t, b, s);    // class 3 has k <= 24
         }

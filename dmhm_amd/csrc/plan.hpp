@@ -170,6 +170,8 @@ struct Plan {
     int batches = 1;        // H*H: number of level-1 subtree batches sharing the workspace
     int64_t wsShared = 0;   // H*H: workspace elements shared by all batches (Omega, T1)
     std::vector<std::shared_ptr<Plan>> members;   // member plans a segmented merged plan reads
+    std::vector<std::unique_ptr<Plan>> chunks;    // merged plans run one after another (a batch
+                                                  // split where the members' workspaces exceed the budget)
     void upload();          // copy tile/contrib/task arrays to the device
     void release();
     ~Plan() { release(); }

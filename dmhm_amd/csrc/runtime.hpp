@@ -175,6 +175,7 @@ dmhm_hmat_s* transpose_hmat(dmhm_hmat_s* s);
 void scale_hmat(dmhm_hmat_s* h, double alpha);
 void add_diag(dmhm_hmat_s* h, double alpha);
 void update_with(dmhm_hmat_s* A, double alpha, dmhm_hmat_s* B, const dmhm_hh_opts& o);
+bool stall_detected(const std::vector<double>& history);
 double estimate_two_norm(dmhm_hmat_s* h, double theta, double confidence, uint64_t seed);
 // HMat2d::DirectInvert (dinv.cpp): A := inv(A) in A's precision.
 void direct_invert(dmhm_hmat_s* A, const dmhm_hh_opts& o);

@@ -237,20 +237,26 @@ typedef struct {
     double switchTol;
     dmhm_hh_opts hh;
     int stallStop;       /* 1: also stop (or leave low precision) once the estimate
-                            is below 0.1 and no longer halves over two iterations */
+                            is below 0.1 and stays above 0.9 of the previous one or
+                            above half of the one two steps back: the truncation
+                            floor. Such a step ends after Z = I - X A and keeps X,
+                            skipping the product Z X */
     int graph;           /* 1: once the plans of a precision are built and their
                             tile engines tuned by eager steps, run each step as a
                             replay of a captured HIP graph (same launches, same
                             seeds; one readback per step for the stop decision) */
     int bf16;            /* mixed mode: the tile GEMMs of the FP32 steps take BF16
                             operands (FP32 accumulation) while the residual
-                            estimate is >= bf16Tol; QR, SVD and the estimate stay
-                            in FP64 arithmetic */
+                            estimate is >= bf16Tol and has not grown by more than
+                            2% over the previous one (after such growth the FP32
+                            steps continue without BF16); QR, SVD and the
+                            estimate stay in FP64 arithmetic */
     double bf16Tol;
     int start;           /* 0: X_0 = alpha A^T (the reference's start); 1: X_0 =
                             the recursive direct inverse of A (dmhm_hmat_direct_invert
-                            with opts->hh, in the precision of the first step), so
-                            the iteration only refines the truncation error */
+                            with opts->hh, in A's precision), so the iteration
+                            only refines the truncation error; every step then
+                            runs in A's precision (mixed has no effect) */
 } dmhm_schulz_opts;
 
 typedef struct {

@@ -50,6 +50,34 @@ def test_direct_invert_exact_without_truncation(n, L, dim, strong, rank):
     assert ranks.max() <= rank
 
 
+def _dense_csr(D):
+    n = D.shape[0]
+    rowptr = np.arange(0, n * n + 1, n, dtype=np.int32)
+    colind = np.tile(np.arange(n, dtype=np.int32), n)
+    return rowptr, colind, np.ascontiguousarray(D, dtype=np.float64).ravel()
+
+
+@pytest.mark.parametrize("sizes", [(1, 1), (5, 1), (17, 1), (33, 1), (8, 8), (10, 8)])
+def test_dense_leaf_inverse_with_pivoting(sizes):
+    """One dense leaf (numLevels 1): a non-symmetric Gaussian matrix with a
+    weak diagonal, so partial pivoting exchanges rows at most steps (m <= 64:
+    the register kernel; 80: the LDS kernel). Against numpy.linalg.inv,
+    FP64 within 1e-9 of max|A^-1| scaled by the condition number's size
+    (cond <= 1e4 here), FP32 within 1e-3."""
+    print("case", sizes, flush=True)
+    n = sizes[0] * sizes[1]
+    rng = np.random.default_rng(n)
+    D = rng.standard_normal((n, n)) + np.diag(1e-3 * np.ones(n))
+    if n > 1:
+        assert np.linalg.cond(D) < 1e4
+    Xr = np.linalg.inv(D)
+    args = _dense_csr(D)
+    for prec, tol in ((dmhm_amd.DMHM_F64, 1e-9), (dmhm_amd.DMHM_F32, 1e-3)):
+        A = dmhm_amd.HMat2d.from_csr([sizes[0], sizes[1]], 1, 4, True, *args, precision=prec)
+        X = A.DirectInvert().to_dense()
+        assert np.abs(X - Xr).max() < tol * np.abs(Xr).max(), (prec, np.abs(X - Xr).max() / np.abs(Xr).max())
+
+
 def test_direct_invert_fp32():
     op, N, args = _op(16, 3, 2)
     A = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 16, True, *args, precision=dmhm_amd.DMHM_F32)
