@@ -10,9 +10,7 @@ X_{k+1} = (2I - X_k A) X_k from the recursive direct inversion
 (HMat2d::DirectInvert) and refines in FP64 (from that start every step runs in
 FP64: it already sits at the truncation floor). Timed once beside it: the
 direct inversion alone, and the reference's own start X_0 = alpha A^T in plain
-FP64 and mixed (FP32 iterations until the estimate is below 1e-3, then FP64),
-the latter also with BF16 tile operands (FP32 accumulation) while the
-estimate is above 0.5. Also reported: H*X on 1 and 64 FP32 right-hand sides
+FP64 and mixed (FP32 iterations until the estimate is below 1e-3, then FP64). Also reported: H*X on 1 and 64 FP32 right-hand sides
 (HBM GB/s, roofline of the accumulation launch), one steady-state H*H product
 (TFLOP/s), and the FP64 CPU oracle on the host cores.
 
@@ -274,9 +272,9 @@ def main():
         X = A.Copy()
         X.SchulzInvert(numIterations=2, start="direct")
         if w == 0:
-            for mixed, bf16 in ((False, False), (True, False), (True, True)):
+            for mixed in (False, True):
                 X = A.Copy()
-                X.SchulzInvert(numIterations=2, mixed=mixed, bf16=bf16)
+                X.SchulzInvert(numIterations=2, mixed=mixed)
     sync()
 
     # The timed step: the inverse to ||I - X A||_2 < stopTol (or the rank
@@ -314,7 +312,6 @@ def main():
     log("bench: direct inversion alone %.3f s, power check %.3e" % (sec_direct, res_direct))
     # The reference's own start X_0 = alpha A^T, plain FP64 and mixed.
     sec64 = rep64 = res64 = sec_ref_mixed = rep_ref_mixed = res_ref_mixed = None
-    sec_ref_bf16 = rep_ref_bf16 = res_ref_bf16 = None
     if not a.no_fp64:
         X64, rep64, sec64 = timed(mixed=False, **sopts)
         res64 = residual_power_estimate(A, X64, n)
@@ -325,11 +322,6 @@ def main():
         log("bench: reference start mixed %.2f s, %d iterations (%d FP32), estimate %.3e, power check %.3e"
             % (sec_ref_mixed, rep_ref_mixed.iterations, rep_ref_mixed.iterationsLowPrecision,
                rep_ref_mixed.residualEstimate, res_ref_mixed))
-        Xb, rep_ref_bf16, sec_ref_bf16 = timed(mixed=True, bf16=True, **sopts)
-        res_ref_bf16 = residual_power_estimate(A, Xb, n)
-        log("bench: reference start mixed+BF16 %.2f s, %d iterations (%d FP32, %d BF16), estimate %.3e, "
-            "power check %.3e" % (sec_ref_bf16, rep_ref_bf16.iterations, rep_ref_bf16.iterationsLowPrecision,
-                                  rep_ref_bf16.iterationsBf16, rep_ref_bf16.residualEstimate, res_ref_bf16))
         v = np.random.default_rng(9).standard_normal(n)
         diff = np.linalg.norm(X.Multiply(1.0, v) - X64.Multiply(1.0, v)) / np.linalg.norm(X.Multiply(1.0, v))
     else:
@@ -403,13 +395,6 @@ def main():
                    "reference_start_mixed_residual_estimate":
                        rep_ref_mixed.residualEstimate if rep_ref_mixed else None,
                    "reference_start_mixed_residual_power_check": res_ref_mixed,
-                   "reference_start_mixed_bf16_seconds": sec_ref_bf16,
-                   "reference_start_mixed_bf16_iterations": rep_ref_bf16.iterations if rep_ref_bf16 else None,
-                   "reference_start_mixed_bf16_bf16_iterations":
-                       rep_ref_bf16.iterationsBf16 if rep_ref_bf16 else None,
-                   "reference_start_mixed_bf16_residual_estimate":
-                       rep_ref_bf16.residualEstimate if rep_ref_bf16 else None,
-                   "reference_start_mixed_bf16_residual_power_check": res_ref_bf16,
                    "speedup_vs_reference_start_fp64": (sec64 / sec) if sec64 else None,
                    "direct_vs_reference_fp64_relative_difference": diff},
         "matvec": {"f32_k1_ms": ms1[0], "f32_k1_GBps": gbs1, "f32_k1_accumulation_GBps": acc_gbs,

@@ -1024,7 +1024,6 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
     }
 
     int it = 0, itLow = 0, itGraph = 0, itBf16 = 0;
-    bool bf16Off = false;
     double e = -1;
     std::vector<double> hist;
     auto stalled = [&]() { return o.stallStop && stall_detected(hist); };
@@ -1041,10 +1040,8 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
         if (low) ++itLow;
         PrecState& P = ps[cur];
         // BF16 operands (FP32 accumulation) for the early low-precision steps:
-        // until the residual estimate drops below bf16Tol, or for good once an
-        // estimate grew by more than 2% over the previous one (near a
-        // residual of 1 the contraction is too weak to absorb BF16 rounding).
-        const int bf = (low && cur == 1 && o.bf16 && !bf16Off && (hist.empty() || std::fabs(e) >= o.bf16Tol)) ? 1 : 0;
+        // until the residual estimate drops below bf16Tol.
+        const int bf = (low && cur == 1 && o.bf16 && (hist.empty() || std::fabs(e) >= o.bf16Tol)) ? 1 : 0;
         if (bf) ++itBf16;
         Bf16Scope bscope(bf != 0);
         SchulzStep st{P.A, Xc, P.X[side].get(), P.Z.get(), stop, o.theta, o.confidence, o.hh,
@@ -1074,7 +1071,6 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
             e = read_two_norm(o.theta);
             if (rep && kk < 128) rep->history[kk] = e;
             hist.push_back(e);
-            if (bf && hist.size() >= 2 && e > 1.02 * hist[hist.size() - 2]) bf16Off = true;
             const bool atFloor = stalled();
             // At the floor the second product cannot improve X: keep X_k.
             if (!low && std::fabs(e) < o.stopTol) done = true;          // the reference's stop (with X_k+1)
@@ -1082,13 +1078,7 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
             else if (low && atFloor) skipTail = true;
         }
         if (!skipTail) {
-            if (bf && bf16Off) {
-                // The step whose estimate grew finishes with FP32 operands.
-                g_lib.bf16 = false;
-                st.tail();
-            } else {
-                half(1);
-            }
+            half(1);
             if (!replay && steady && g_lib.untuned == u0) P.tuned[bf] = true;
             Xc = P.X[side].get();
             side ^= 1;

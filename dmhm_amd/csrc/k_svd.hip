@@ -8,12 +8,13 @@ namespace dmhm {
 // R = U S V^T; outputs G1 = V S (first `keep` columns, rest zero) and
 // G2 = U (same), both k x cap, and the kept rank.
 // ---------------------------------------------------------------------------
-// KMAX bounds k (LDS sized for it); NT threads, 8 per column pair.
-template <typename T, int KMAX, int NT>
+// KMAX bounds k (LDS sized for it); NT threads, TPP (a power of two <= 8)
+// per column pair.
+template <typename T, int KMAX, int NT, int TPP = 8>
 __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restrict__ tasks,
                                                  int tolMode, double relTol)
 {
-    static_assert(NT >= 8 * (KMAX / 2), "one 8-thread group per column pair");
+    static_assert(NT >= TPP * (KMAX / 2), "one TPP-thread group per column pair");
     __shared__ double As[KMAX][KMAX + 1];
     __shared__ double Vs[KMAX][KMAX + 1];
     __shared__ double sig[KMAX];
@@ -47,7 +48,9 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
         if ((tid & 63) == 0) atomicAdd(&fro2, s);
     }
     const int kp = k + (k & 1);
-    const int pi = tid >> 3, sub = tid & 7;
+    constexpr int LOG = TPP == 8 ? 3 : (TPP == 4 ? 2 : (TPP == 2 ? 1 : 0));
+    static_assert((1 << LOG) == TPP, "TPP must be 1, 2, 4 or 8");
+    const int pi = tid >> LOG, sub = tid & (TPP - 1);
     for (int sweep = 0; sweep < 60; ++sweep) {
         __syncthreads();
         if (tid == 0) rotated = 0;
@@ -59,12 +62,12 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
                 const int p = min(a0, a1), q = max(a0, a1);
                 double al = 0, be = 0, ga = 0;
                 if (q < k)
-                    for (int r = sub; r < k; r += 8) {
+                    for (int r = sub; r < k; r += TPP) {
                         const double x = As[p][r], y = As[q][r];
                         al += x * x; be += y * y; ga += x * y;
                     }
 #pragma unroll
-                for (int off = 4; off >= 1; off >>= 1) {
+                for (int off = TPP / 2; off >= 1; off >>= 1) {
                     al += __shfl_xor(al, off, 64);
                     be += __shfl_xor(be, off, 64);
                     ga += __shfl_xor(ga, off, 64);
@@ -73,7 +76,7 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
                     const double zeta = (be - al) / (2.0 * ga);
                     const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
-                    for (int r = sub; r < k; r += 8) {
+                    for (int r = sub; r < k; r += TPP) {
                         const double x = As[p][r], y = As[q][r];
                         As[p][r] = cs * x - sn * y; As[q][r] = sn * x + cs * y;
                         const double vx = Vs[p][r], vy = Vs[q][r];

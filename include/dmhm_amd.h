@@ -247,10 +247,12 @@ typedef struct {
                             seeds; one readback per step for the stop decision) */
     int bf16;            /* mixed mode: the tile GEMMs of the FP32 steps take BF16
                             operands (FP32 accumulation) while the residual
-                            estimate is >= bf16Tol and has not grown by more than
-                            2% over the previous one (after such growth the FP32
-                            steps continue without BF16); QR, SVD and the
-                            estimate stay in FP64 arithmetic */
+                            estimate is >= bf16Tol; QR, SVD and the estimate
+                            stay in FP64 arithmetic. From X_0 = alpha A^T on an
+                            ill-conditioned operator BF16 rounding can turn the
+                            smallest eigenvalues of X A negative, from where the
+                            iteration diverges (the 256^2 lognormal operator
+                            does): use it on well-conditioned operators only */
     double bf16Tol;
     int start;           /* 0: X_0 = alpha A^T (the reference's start); 1: X_0 =
                             the recursive direct inverse of A (dmhm_hmat_direct_invert

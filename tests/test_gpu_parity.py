@@ -249,6 +249,26 @@ def test_schulz_exact_rank_reaches_roundoff():
     assert res < 1e-9
 
 
+def test_schulz_bf16_on_well_conditioned_operator():
+    """Mixed Schulz with BF16 tile operands in the early FP32 steps (the
+    iteration's own quadratic convergence absorbs their rounding) on the
+    smooth-coefficient 32^2 operator: it ends where the FP32 mixed form ends,
+    within 10x on the dense residual ||I - A X||_2."""
+    op = W.small_operator(32, 3)
+    D = W.csr_to_dense(*_args(op), 1024)
+    out = {}
+    for bf16 in (False, True):
+        H = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 16, True, *_args(op))
+        rep = H.SchulzInvert(stopTol=1e-8, maxIterations=60, mixed=True, bf16=bf16, stall_stop=True)
+        r = np.linalg.norm(np.eye(1024) - D @ H.to_dense(), 2)
+        print("bf16", bf16, "iterations", rep.iterations, "bf16 steps", rep.iterationsBf16,
+              "estimate", rep.residualEstimate, "dense residual", r, flush=True)
+        assert np.isfinite(rep.residualEstimate)
+        out[bf16] = (rep, r)
+    assert out[True][0].iterationsBf16 > 0
+    assert out[True][1] <= 10 * out[False][1] + 1e-12
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_schulz_graph_replay_matches_eager(mixed):
     """Steps replayed from captured HIP graphs launch what the eager steps
