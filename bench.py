@@ -139,30 +139,45 @@ def cpu_baseline(sample, iterations):
 
 
 def batch_mode(a, np, dmhm_amd, W):
-    """The batch of independent 3D operators (-div(a grad u) + u, a = 1 + 0.5
-    U[0,1) per cell, HMat3d numLevels 4, maxRank 32, FP32; sides drawn from
-    --batch-sides): --batch-iters Schulz steps from X_0 = alpha A^T, all
-    members in one launch per phase, then the same steps member by member.
-    Each form runs once untimed (plans, workspace, tile-engine tuning) and
-    once timed."""
+    """A batch of independent operators -div(a grad u) + u, FP32: by default
+    3D (a = 1 + 0.5 U[0,1) per cell, HMat3d numLevels 4, maxRank 32, sides
+    drawn from --batch-sides); with --batch-dim 2, 2D lognormal operators
+    (HMat2d numLevels 4, maxRank 8, one seed per member). --batch-iters Schulz
+    steps from X_0 = alpha A^T, the members batched (merged launches where
+    they pay), then the same steps member by member. Each form runs once
+    untimed (plans, workspace, tile-engine tuning) and once timed."""
     sync = dmhm_amd.load().dmhm_synchronize
     sides = tuple(int(s) for s in a.batch_sides.split(","))
-    log("bench: building %d 3D operators (sides %s)" % (a.batch_count, sides))
-    ops = W.c3_batch(a.batch_count, sides=sides)
-    mats = [dmhm_amd.HMat3d.from_csr(o["sizes"], 4, 32, True, o["rowptr"], o["colind"], o["vals"],
-                                     precision=dmhm_amd.DMHM_F32) for o in ops]
+    log("bench: building %d %dD operators (sides %s)" % (a.batch_count, a.batch_dim, sides))
+    if a.batch_dim == 3:
+        ops = W.c3_batch(a.batch_count, sides=sides)
+        mats = [dmhm_amd.HMat3d.from_csr(o["sizes"], 4, 32, True, o["rowptr"], o["colind"], o["vals"],
+                                         precision=dmhm_amd.DMHM_F32) for o in ops]
+    else:
+        rng = np.random.default_rng(0)
+        ops = [W.small_operator(int(s), 4, dim=2, smooth=False, seed=i)
+               for i, s in enumerate(rng.choice(np.asarray(sides), size=a.batch_count))]
+        mats = [dmhm_amd.HMat2d.from_csr(o["sizes"], 4, 8, True, o["rowptr"], o["colind"], o["vals"],
+                                         precision=dmhm_amd.DMHM_F32) for o in ops]
     K = a.batch_iters
-    out = {"mode": "batch", "members": len(mats), "sides": [int(o["sizes"][0]) for o in ops],
-           "unknowns": int(sum(np.prod(o["sizes"]) for o in ops)), "schulz_steps": K, "precision": "f32"}
+    G = max(1, a.batch_group)
+    groups = [mats[i:i + G] for i in range(0, len(mats), G)]
+    out = {"mode": "batch", "dim": a.batch_dim, "members": len(mats), "sides": [int(o["sizes"][0]) for o in ops],
+           "unknowns": int(sum(np.prod(o["sizes"]) for o in ops)), "schulz_steps": K, "precision": "f32",
+           "batches": len(groups), "members_per_batch": G,
+           "operator_storage_gb": sum(m.info().storageBytes for m in mats) / 1e9}
     for tag in ("untimed", "timed"):
-        B = dmhm_amd.HBatch.create(mats)
-        sync()
-        t = time.perf_counter()
-        B.SchulzInvert(numIterations=K)
-        sync()
-        out["batched_%s_s" % tag] = time.perf_counter() - t
-        log("bench: batched %s %.3f s" % (tag, out["batched_%s_s" % tag]))
-        del B
+        total = 0.0
+        for grp in groups:
+            B = dmhm_amd.HBatch.create(grp)
+            sync()
+            t = time.perf_counter()
+            B.SchulzInvert(numIterations=K)
+            sync()
+            total += time.perf_counter() - t
+            del B
+        out["batched_%s_s" % tag] = total
+        log("bench: batched %s %.3f s (%d batches)" % (tag, total, len(groups)))
     for tag in ("untimed", "timed"):
         sync()
         t = time.perf_counter()
@@ -193,6 +208,11 @@ def main():
     ap.add_argument("--batch-count", type=int, default=48)
     ap.add_argument("--batch-sides", default="16,20,24,28,32")
     ap.add_argument("--batch-iters", type=int, default=2, help="Schulz steps of the batch mode")
+    ap.add_argument("--batch-dim", type=int, choices=[2, 3], default=3)
+    ap.add_argument("--batch-group", type=int, default=16,
+                    help="members per batch: a batched Schulz holds six copies of its members (operator, "
+                         "two iterates, Z, start, result), so 48 members of 16^3-32^3 at maxRank 32 "
+                         "(0.16-1.45 GB each in FP32) run as three batches of 16")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--config", choices=["c2", "c4"], default="c2",
                     help="c2: 2D 256^2 (the headline workload); c4: 2D 512^2, sigma=1.5, l=0.05, "

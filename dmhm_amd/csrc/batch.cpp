@@ -120,6 +120,15 @@ static void build_parallel(size_t count, const std::function<void(size_t)>& buil
 }
 
 // Merged plan of C_i := alpha A_i B_i (mode: plain product) for every member.
+// Members whose product has a tile phase of at least this many tiles run
+// unmerged (a chunk of their own).
+constexpr size_t kAloneTiles = 65536;
+
+// Member (class) product plans, shared by every batch whose members have the
+// same geometry, layouts and operand forms: built, uploaded and tile-tuned
+// once. (Allocated once and released by dmhm_finalize.)
+static std::map<std::string, std::shared_ptr<Plan>>* g_classPlans = new std::map<std::string, std::shared_ptr<Plan>>();
+
 static Plan* merged_product(const Arena& A, const Arena& B, const Arena& C, int oversample)
 {
     const size_t M = A.size();
@@ -143,31 +152,63 @@ static Plan* merged_product(const Arena& A, const Arena& B, const Arena& C, int 
     // budget, so the same level-1 batches and phases): members merge phase
     // for phase, and a batch launches what one member launches.
     const int64_t budget = hh_batch_budget();
-    build_parallel(cls.size(), [&](size_t k) {
-        Cls& x = cls[k];
+    std::vector<std::string> ckey(cls.size());
+    std::vector<size_t> todo;
+    for (size_t k = 0; k < cls.size(); ++k) {
+        const Cls& x = cls[k];
+        ckey[k] = std::to_string(reinterpret_cast<uintptr_t>(x.a->gc.get())) + ":" + std::to_string(x.a->lay->id) + "," +
+                  std::to_string(x.b->lay->id) + "," + std::to_string(x.c->lay->id) + ":" +
+                  std::to_string((int)x.a->full) + std::to_string((int)x.b->full) + ":" +
+                  std::to_string(x.c->maxRank) + ":" + std::to_string(oversample);
+        auto it = g_classPlans->find(ckey[k]);
+        if (it != g_classPlans->end()) cls[k].plan = it->second;
+        else todo.push_back(k);
+    }
+    build_parallel(todo.size(), [&](size_t i) {
+        Cls& x = cls[todo[i]];
         Planner pl(x.a->geo());
         pl.setWorkspaceBudget(budget);
         pl.setStacking(x.a->full, x.b->full);
         x.plan.reset(pl.hhProduct(*x.a->lay, *x.b->lay, *x.c->lay, x.c->maxRank, oversample));
+        x.plan->keepHost = true;
     });
     std::unique_ptr<Plan> holder(new Plan());
-    // Chunks of consecutive members whose workspaces together fit the budget
+    // Chunks of members of one class whose workspaces together fit the budget
     // (at least one member each), one merged plan per chunk; the chunks run
-    // one after another in one shared workspace.
+    // one after another in one shared workspace. Members of a class share
+    // their plan, so a chunk's merged phases are the class plan's phases and
+    // its tiles keep the class's shapes (a phase's engine is instantiated for
+    // its largest tile, which a smaller member would otherwise run at).
+    std::vector<size_t> order(M);
+    for (size_t m = 0; m < M; ++m) order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return of[x] < of[y]; });
+    // A member whose tile phases fill the chip many times over runs as a
+    // chunk of its own: merging would interleave the members phase by phase,
+    // and a phase's output (hundreds of MB) would leave the caches before
+    // the next phase of the same member reads it. (C3's 3D operators carry
+    // 0.8-1M tiles in their largest phase at every size from 8^3 to 32^3; 48
+    // of them in chunks of several members took 1.34x the time of one by
+    // one.)
+    std::vector<char> alone(cls.size(), 0);
+    for (size_t k = 0; k < cls.size(); ++k)
+        for (const Phase& ph : cls[k].plan->phases)
+            if (ph.kind == PH_TILE && ph.tiles.size() >= kAloneTiles) alone[k] = 1;
     const int64_t slots[NUM_SLOTS] = {A.totalVals, B.totalVals, C.totalVals, 0, 0, 0, 0, 0};
     const int64_t ranks[NUM_SLOTS] = {A.totalRanks, B.totalRanks, C.totalRanks, 0, 0, 0, 0, 0};
-    for (size_t m0 = 0; m0 < M;) {
+    for (size_t i0 = 0; i0 < M;) {
         int64_t ws = 0;
-        size_t m1 = m0;
-        while (m1 < M) {
-            const int64_t w = (cls[of[m1]].plan->wsElems + 63) & ~int64_t(63);
-            if (m1 > m0 && ws + w > budget) break;
+        size_t i1 = i0;
+        while (i1 < M) {
+            const size_t m = order[i1];
+            const int64_t w = (cls[of[m]].plan->wsElems + 63) & ~int64_t(63);
+            if (i1 > i0 && (ws + w > budget || of[m] != of[order[i0]] || alone[of[m]])) break;
             ws += w;
-            ++m1;
+            ++i1;
         }
-        std::vector<PlanPart> parts(m1 - m0);
-        for (size_t m = m0; m < m1; ++m) {
-            PlanPart& p = parts[m - m0];
+        std::vector<PlanPart> parts(i1 - i0);
+        for (size_t i = i0; i < i1; ++i) {
+            const size_t m = order[i];
+            PlanPart& p = parts[i - i0];
             p.plan = cls[of[m]].plan.get();
             p.val[SLOT_A] = A.val[m]; p.rank[SLOT_A] = A.rank[m];
             p.val[SLOT_B] = B.val[m]; p.rank[SLOT_B] = B.rank[m];
@@ -181,15 +222,16 @@ static Plan* merged_product(const Arena& A, const Arena& B, const Arena& C, int 
         holder->wsElems = std::max(holder->wsElems, merged->wsElems);
         holder->flops += merged->flops;
         holder->chunks.push_back(std::move(merged));
-        m0 = m1;
+        i0 = i1;
     }
-    // The checks above read the class plans' host arrays, which an upload
-    // releases; segmented tile phases then read the class plans' arrays on
-    // the device, so the class plans go up first.
-    for (auto& x : cls) {
-        x.plan->upload();
-        holder->members.push_back(x.plan);
+    // Segmented tile phases read the class plans' arrays on the device, so
+    // new class plans go up first (keeping their host arrays, which later
+    // merged plans are checked against).
+    for (size_t k : todo) {
+        cls[k].plan->upload();
+        (*g_classPlans)[ckey[k]] = cls[k].plan;
     }
+    for (auto& x : cls) holder->members.push_back(x.plan);
     for (auto& c : holder->chunks) c->upload();
     return holder.release();
 }
@@ -335,7 +377,11 @@ static void read_batch_two_norm(dmhm_hbatch_s& B, double theta, std::vector<doub
 // that could run after the HIP runtime is gone.)
 static std::map<std::string, std::unique_ptr<Plan>>* g_batchPlans = new std::map<std::string, std::unique_ptr<Plan>>();
 
-void release_batch_plans() { g_batchPlans->clear(); }
+void release_batch_plans()
+{
+    g_batchPlans->clear();
+    g_classPlans->clear();
+}
 
 static Plan& batch_plan(dmhm_hbatch_s& B, const std::string& key, const std::function<Plan*()>& make)
 {

@@ -170,7 +170,9 @@ static const int kTuneCands[3][5] = {{TILE_LDS, TILE_MFMA, -1, -1, -1},
 static const int kTuneCount[3] = {2, 3, 5};
 
 // Launches a TILE phase with its engine. Autotuning (no engine forced): the
-// first runs of a phase in a slot time one candidate each (a complete,
+// first run of a phase in a slot is a warm-up with the first candidate (its
+// operands' first touch and cold caches would otherwise be charged to that
+// candidate alone), the next runs time one candidate each (a complete,
 // correct execution every time, synchronised for the timing), after which the
 // fastest candidate is kept for that slot.
 template <typename T>
@@ -184,9 +186,23 @@ static void run_tile_phase(Phase& ph, const Bases& b, T alpha, T beta)
         launch_tile_phase<T>(ph, b, alpha, beta, s, e);
         return;
     }
-    Phase::Tune& t = ph.tune[slot];
+    // Segments of one source phase tune in that phase, timed per segment.
+    const Phase* tp = &ph;
+    if (!ph.segSrc.empty()) {
+        tp = ph.segSrc[0].src;
+        for (const auto& sg : ph.segSrc)
+            if (sg.src != tp) { tp = &ph; break; }
+    }
+    const float per = ph.segSrc.empty() || tp == &ph ? 1.0f : (float)ph.segSrc.size();
+    Phase::Tune& t = tp->tune[slot];
     if (t.choice >= 0) {
         launch_tile_phase<T>(ph, b, alpha, beta, s, t.choice);
+        return;
+    }
+    if (t.next < 0) {
+        launch_tile_phase<T>(ph, b, alpha, beta, s, kTuneCands[slot][0]);
+        ++g_lib.untuned;
+        t.next = 0;
         return;
     }
     const int cand = kTuneCands[slot][t.next];
@@ -195,7 +211,7 @@ static void run_tile_phase(Phase& ph, const Bases& b, T alpha, T beta)
     launch_tile_phase<T>(ph, b, alpha, beta, s, cand);
     HIPCHECK(hipEventRecord(ev.b, s));
     HIPCHECK(hipEventSynchronize(ev.b));
-    t.ms[t.next] = ev.ms();
+    t.ms[t.next] = ev.ms() / per;
     ++g_lib.untuned;
     if (++t.next == kTuneCount[slot]) {
         int best = 0;
@@ -2383,8 +2399,9 @@ void Plan::upload()
             HIPCHECK(hipMalloc(&ph.dTasks, bytes));
             HIPCHECK(copy_sync(ph.dTasks, src, bytes, hipMemcpyHostToDevice));
         }
-        // Host copies of the big arrays are no longer needed.
-        std::vector<Contrib>().swap(ph.contribs);
+        // Host copies of the big arrays are no longer needed (unless merged
+        // plans will be checked against them).
+        if (!keepHost) std::vector<Contrib>().swap(ph.contribs);
     }
 }
 

@@ -250,95 +250,10 @@ __global__ void __launch_bounds__(64 * (64 / RW)) dense_inverse_reg_kernel(const
         if (c < m) dst[c + (int64_t)j * m] = (T)tile[sigma[j] * LD + c];
 }
 
-// The same with one barrier per step: every wave publishes, with its pivot
-// candidate, its candidate row (and the owner of row k that row), so the
-// chosen pivot row is already in LDS when the candidates are compared. The
-// LDS buffers alternate between steps.
-template <typename T, int RW>
-__global__ void __launch_bounds__(64 * (64 / RW)) dense_inverse_reg1_kernel(const T* __restrict__ src,
-                                                                           T* __restrict__ dst, int m,
-                                                                           int* __restrict__ singular)
-{
-    constexpr int NWV = 64 / RW, LD = 65;
-    __shared__ double tile[64 * LD];          // tile[j * LD + i] = a(i, j)
-    __shared__ double crow[2][NWV][64], krow[2][64];
-    __shared__ double cval[2][NWV];
-    __shared__ int cidx[2][NWV];
-    __shared__ int sigma[64];
-    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6, r0 = g * RW;
-    for (int j = g; j < 64; j += NWV)
-        tile[j * LD + c] = (j < m && c < m) ? (double)src[c + (int64_t)j * m] : 0.0;
-    __syncthreads();
-    double a[RW];
-#pragma unroll
-    for (int r = 0; r < RW; ++r) a[r] = tile[c * LD + r0 + r];
-    int x = c;                                // source column of result column c
-#pragma unroll 1
-    for (int k = 0; k < m; ++k) {
-        const int buf = k & 1;
-        double best = -1.0;
-        int bi = 0;
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int i = r0 + r;
-            const double v = (i >= k && i < m) ? fabs(a[r]) : -1.0;
-            if (v > best) { best = v; bi = i; }
-        }
-        best = bcast(best, k);
-        bi = __builtin_amdgcn_readlane(bi, k);
-        if (c == 0) { cval[buf][g] = best; cidx[buf][g] = bi; }
-        {
-            double v = 0.0;
-#pragma unroll
-            for (int r = 0; r < RW; ++r) v = (r0 + r == bi) ? a[r] : v;
-            crow[buf][g][c] = v;
-        }
-        const int gk = k / RW, rk = k % RW;
-        if (g == gk) {
-            double v = 0.0;
-#pragma unroll
-            for (int r = 0; r < RW; ++r) v = (r == rk) ? a[r] : v;
-            krow[buf][c] = v;
-        }
-        __syncthreads();
-        double pb = cval[buf][0];
-        int p = cidx[buf][0], gp = 0;
-#pragma unroll
-        for (int q = 1; q < NWV; ++q)
-            if (cval[buf][q] > pb) { pb = cval[buf][q]; p = cidx[buf][q]; gp = q; }
-        p = __builtin_amdgcn_readfirstlane(p);
-        gp = __builtin_amdgcn_readfirstlane(gp);
-        const double d = crow[buf][gp][k];    // the pivot
-        const double fp = krow[buf][k];       // row p's new entry in column k (old a(k,k))
-        const double oldk = krow[buf][c];     // row p's new entry in column c
-        if (d == 0.0 && tid == 0) atomicExch(singular, 1);
-        const double inv = d == 0.0 ? 0.0 : 1.0 / d;
-        const double rkc = crow[buf][gp][c] * inv;   // pivot row, scaled
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int i = r0 + r;
-            const double ck = bcast(a[r], k);  // this row's entry in column k
-            const bool swapped = (i == p) && (p != k);
-            const double f = swapped ? fp : ck;
-            const double cur = swapped ? oldk : a[r];
-            double v;
-            if (i == k) v = (c == k) ? inv : rkc;
-            else v = (c == k) ? -f * inv : cur - f * rkc;
-            a[r] = v;
-        }
-        x = (x == k) ? p : ((x == p) ? k : x);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RW; ++r) tile[c * LD + r0 + r] = a[r];
-    sigma[c] = x;
-    __syncthreads();
-    for (int j = g; j < m; j += NWV)
-        if (c < m) dst[c + (int64_t)j * m] = (T)tile[sigma[j] * LD + c];
-}
-
 // Rows per wave of the register inversion (8 waves of 8 rows: 108 us for a
-// 64 x 64 leaf against 122 us with 4 waves and 166 us with 16).
+// 64 x 64 leaf against 122 us with 4 waves, 166 us with 16 and 178 us for
+// the LDS kernel with 512 threads; publishing the candidate rows to save the
+// second barrier gained 3%).
 constexpr int kInvRows = 8;
 
 template <typename T>

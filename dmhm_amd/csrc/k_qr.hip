@@ -418,16 +418,18 @@ static void launch_qr_classes(const QrTask* tasks, const int* classEnd, const Ba
         const int n = classEnd[c] - begin;
         if (n > 0) {
             const QrTask* t = tasks + begin;
-            // Few panels (latency): the column-per-lane kernel with more waves
-            // than rows per wave; many panels (throughput): the row-per-lane
-            // kernel, except for wide panels of <= 64 rows.
+            // Measured per class (FP64, k = 20 and 36, 80 and 2,000 panels):
+            // few panels (latency) of <= 64 rows and few wide panels of <= 256
+            // rows take the column-per-lane kernel; otherwise the row-per-lane
+            // kernel (with batched column sums for few 24-column panels),
+            // except for many wide panels of <= 64 rows.
             const bool few = n < kQrLatencyPanels;
             if (c == 0) {
                 if (few || KMAX > 24) qr_col_kernel<T, 8, 8><<<n, 512, 0, s>>>(b, t);
                 else qr_reg_kernel<T, KMAX, 1, false><<<n, 64, 0, s>>>(b, t);
             } else if (c == 1) {
-                if (few) qr_col_kernel<T, 16, 16><<<n, 1024, 0, s>>>(b, t);
-                else qr_reg_kernel<T, KMAX, 4, false><<<n, 256, 0, s>>>(b, t);
+                if (few && KMAX > 24) qr_col_kernel<T, 16, 16><<<n, 1024, 0, s>>>(b, t);
+                else launch_qr_reg<T, KMAX, 4>(n, t, b, s);
             }
             else if (c == 2) launch_qr_reg<T, KMAX, 8>(n, t, b, s);
             else launch_qr_reg<T, 24, 16>(n, t, b, s);    // class 3 has k <= 24

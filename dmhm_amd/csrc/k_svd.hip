@@ -66,12 +66,9 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
                         const double x = As[p][r], y = As[q][r];
                         al += x * x; be += y * y; ga += x * y;
                     }
-#pragma unroll
-                for (int off = TPP / 2; off >= 1; off >>= 1) {
-                    al += __shfl_xor(al, off, 64);
-                    be += __shfl_xor(be, off, 64);
-                    ga += __shfl_xor(ga, off, 64);
-                }
+                al = group_sum<TPP>(al);
+                be = group_sum<TPP>(be);
+                ga = group_sum<TPP>(ga);
                 if (q < k && fabs(ga) > 1e-15 * sqrt(al * be) && fabs(ga) > 1e-30 * fro2) {
                     const double zeta = (be - al) / (2.0 * ga);
                     const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));

@@ -19,25 +19,66 @@ namespace dmhm {
             throw std::runtime_error(std::string("HIP launch: ") + hipGetErrorString(e_)); \
     } while (0)
 
+// Cross-lane moves inside rows of 16 lanes by DPP (VALU, no LDS round trip):
+// the xor-1 and xor-2 quad permutations and the mirrors within 8 and 16
+// lanes. After an xor-1 and an xor-2 step every lane of a quad holds the
+// quad's sum; a mirror then pairs each lane with one of the other half, so
+// each step adds the same two partial sums on both sides (a + b and b + a)
+// and every lane of a group ends with the bitwise same sum.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppMirror8 = 0x141, kDppMirror16 = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over aligned groups of G lanes (G = 1, 2, 4, 8 or 16); every lane of a
+// group receives it. The whole group must be active.
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v)
+{
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16, "group of 1, 2, 4, 8 or 16 lanes");
+    if constexpr (G >= 2) v += dpp_mov<kDppXor1>(v);
+    if constexpr (G >= 4) v += dpp_mov<kDppXor2>(v);
+    if constexpr (G >= 8) v += dpp_mov<kDppMirror8>(v);
+    if constexpr (G >= 16) v += dpp_mov<kDppMirror16>(v);
+    return v;
+}
+
+// Wave sums (all 64 lanes active): DPP inside each row of 16, then two
+// shuffles across rows.
 __device__ inline double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    v = group_sum<16>(v);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
     return v;
 }
 __device__ inline float wave_sum(float v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    v = group_sum<16>(v);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
     return v;
 }
-// Wave sums of N values at once: each butterfly step issues the N shuffles
-// together, so their latency is paid once per step instead of once per value.
+// Wave sums of N values at once: the DPP steps of all values, then each
+// cross-row step issues the N shuffles together, so their latency is paid
+// once per step instead of once per value.
 template <int N>
 __device__ __forceinline__ void wave_sum_all(double (&v)[N])
 {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
+    for (int c = 0; c < N; ++c) v[c] = group_sum<16>(v[c]);
+#pragma unroll
+    for (int off = 16; off <= 32; off <<= 1) {
         double t[N];
 #pragma unroll
         for (int c = 0; c < N; ++c) t[c] = __shfl_xor(v[c], off, 64);

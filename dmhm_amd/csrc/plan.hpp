@@ -156,10 +156,13 @@ struct Phase {
     int role = 0;           // 0 accumulation, 1 projection (profiling identity)
     double flops = 0;       // algorithmic flops (2 m q k per contribution at static sizes)
     // Tile-engine autotuning state per execution slot (FP64, FP32, FP32 with
-    // BF16 operands allowed): candidates timed so far and the choice (-1 while
-    // tuning).
-    struct Tune { int choice = -1; int next = 0; float ms[8] = {0, 0, 0, 0, 0, 0, 0, 0}; };
-    Tune tune[3];
+    // BF16 operands allowed): candidates timed so far (per member segment)
+    // and the choice (-1 while tuning). A merged phase whose segments all
+    // come from one member plan's phase tunes in that phase (mutable: merged
+    // plans hold their sources const), so every chunk of a class shares one
+    // tuning, as the members' lone runs share their class plan's.
+    struct Tune { int choice = -1; int next = -1; float ms[8] = {0, 0, 0, 0, 0, 0, 0, 0}; };   // next -1: warm-up run
+    mutable Tune tune[3];
 };
 
 struct Plan {
@@ -169,6 +172,8 @@ struct Plan {
     int64_t contribCount = 0, tileCount = 0;
     int batches = 1;        // H*H: number of level-1 subtree batches sharing the workspace
     int64_t wsShared = 0;   // H*H: workspace elements shared by all batches (Omega, T1)
+    bool keepHost = false;  // keep the host tile/contribution arrays after upload (plans that
+                            // merged plans are built from and checked against)
     std::vector<std::shared_ptr<Plan>> members;   // member plans a segmented merged plan reads
     std::vector<std::unique_ptr<Plan>> chunks;    // merged plans run one after another (a batch
                                                   // split where the members' workspaces exceed the budget)
