@@ -1340,7 +1340,9 @@ static void profile_matvec(dmhm_hmat_s* h, int k, int reps, double* ms, double* 
     Bases b = bases_for(h, nullptr, nullptr, g_lib.stageX.ptr, g_lib.stageY.ptr);
     EventPair e0, e1;
     double tot = 0, t0 = 0, t1 = 0;
-    for (int r = -2; r < reps; ++r) {
+    // Untimed runs first: enough for the tile-engine tuning of the non-
+    // streaming form (a warm-up and up to five candidates).
+    for (int r = -6; r < reps; ++r) {
         HIPCHECK(hipEventRecord(e0.a, g_lib.stream));
         for (size_t i = 0; i < last; ++i) {
             if (h->prec == DMHM_F64) run_phase<double>(p.phases[i], b, 1.0, 0.0, {0, 0}, 0);
