@@ -913,6 +913,41 @@ static void launch_two_norm_t(dmhm_hmat_s* h, double confidence, double theta, u
     }
 }
 
+// The same estimate for I - X A applied through H*x products, (I - X A) v =
+// v - X (A v), without forming the product X A.
+template <typename T>
+static void launch_residual_norm_t(dmhm_hmat_s* X, dmhm_hmat_s* A, double confidence, double theta, uint64_t seed)
+{
+    const int n = X->geo().n;
+    const int k = (int)std::ceil(std::log(0.8 * std::sqrt((double)n) * std::pow(10.0, confidence)) /
+                                 std::log(theta));
+    g_lib.stageX.ensure(3 * (size_t)n * sizeof(double) + 64);
+    T* x = (T*)g_lib.stageX.ptr;
+    T* y = x + n;
+    T* w = y + n;
+    hipStream_t st = g_lib.stream;
+    double* d = g_lib.dScalars;       // d[0] sumsq, d[1] log accumulator
+    launch_gaussian<T>(x, n, seed, 7, st, g_lib.seedOff);
+    launch_zero_double(d, 2, st);
+    launch_sumsq<T>(x, n, d, st);
+    launch_normalize<T>(x, n, d, nullptr, 0.0, st);
+    for (int i = 0; i < k; ++i) {
+        multiply_dense(A, false, 1.0, x, n, 1, 0.0, y, n);
+        multiply_dense(X, false, 1.0, y, n, 1, 0.0, w, n);
+        launch_axpby<T>(x, w, T(-1), y, n, st);
+        launch_zero_double(d, 1, st);
+        launch_sumsq<T>(y, n, d, st);
+        launch_normalize<T>(y, n, d, d + 1, 1.0 / k, st);
+        std::swap(x, y);
+    }
+}
+
+static void launch_residual_norm(dmhm_hmat_s* X, dmhm_hmat_s* A, double theta, double confidence, uint64_t seed)
+{
+    if (X->prec == DMHM_F64) launch_residual_norm_t<double>(X, A, confidence, theta, seed);
+    else launch_residual_norm_t<float>(X, A, confidence, theta, seed);
+}
+
 static void launch_two_norm(dmhm_hmat_s* h, double theta, double confidence, uint64_t seed)
 {
     if (h->prec == DMHM_F64) launch_two_norm_t<double>(h, confidence, theta, seed);
@@ -1050,7 +1085,33 @@ static void schulz(dmhm_hmat_s* A, const dmhm_schulz_opts& o, dmhm_schulz_report
         explicit Bf16Scope(bool on) { g_lib.bf16 = on; }
         ~Bf16Scope() { g_lib.bf16 = false; }
     };
-    for (int kk = 0; kk < numIt; ++kk) {
+    // From the direct start the iterate usually sits at the truncation floor
+    // already. Each step then first estimates ||I - X_k A||_2 through H*x
+    // products (the same estimator, applied to the operator instead of the
+    // compressed Z) and decides before Z = I - X_k A is formed: at the floor
+    // the step ends there, keeping X_k; below the tolerance it takes the
+    // reference's last refinement step and stops.
+    const bool direct = o.start == 1 && stop;
+    if (direct) {
+        PrecState& P = ps[cur];
+        for (int kk = 0; kk < numIt; ++kk) {
+            it = kk + 1;
+            launch_residual_norm(Xc, P.A, o.theta, o.confidence, seed + 1 + 3 * (uint64_t)kk);
+            e = read_two_norm(o.theta);
+            if (rep && kk < 128) rep->history[kk] = e;
+            hist.push_back(e);
+            const bool conv = std::fabs(e) < o.stopTol;
+            if (!conv && stalled()) break;
+            SchulzStep st{P.A, Xc, P.X[side].get(), P.Z.get(), false, o.theta, o.confidence, o.hh,
+                          seed + 1 + 3 * (uint64_t)cur};
+            st.head();
+            st.tail();
+            Xc = P.X[side].get();
+            side ^= 1;
+            if (conv) break;
+        }
+    }
+    for (int kk = 0; kk < (direct ? 0 : numIt); ++kk) {
         it = kk + 1;
         const bool low = cur != target;
         if (low) ++itLow;

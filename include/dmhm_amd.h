@@ -258,7 +258,10 @@ typedef struct {
                             the recursive direct inverse of A (dmhm_hmat_direct_invert
                             with opts->hh, in A's precision), so the iteration
                             only refines the truncation error; every step then
-                            runs in A's precision (mixed has no effect) */
+                            runs in A's precision (mixed has no effect) and, in
+                            stop mode, begins with the estimate of ||I - X A||_2
+                            applied through H*x products, so a step at the
+                            truncation floor ends before Z = I - X A is formed */
 } dmhm_schulz_opts;
 
 typedef struct {
