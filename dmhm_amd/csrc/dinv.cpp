@@ -58,9 +58,16 @@ struct DinvProgram {
     std::vector<DinvStep> steps;
     std::vector<CopyRange> ranges;
     DeviceBuffer dRanges;
+    DeviceBuffer arenaV, arenaR, flag;   // the arena (X, W, K) and the singular flag, kept between calls
     long products = 0, plans = 0;   // sub-matrix products, merged plans
     double buildSeconds = 0;
-    ~DinvProgram() { dRanges.release(); }
+    ~DinvProgram()
+    {
+        dRanges.release();
+        arenaV.release();
+        arenaR.release();
+        flag.release();
+    }
 };
 
 namespace {
@@ -358,15 +365,16 @@ void direct_invert(dmhm_hmat_s* A, const dmhm_hh_opts& o)
     const size_t es = A->esize();
     g_lib.ws.ensure(std::max<size_t>(P->wsElems * es, 256));
     hipStream_t s = g_lib.stream;
-    DeviceBuffer arenaV, arenaR, flag;
-    arenaV.ensure(3 * P->memberElems * es);
-    arenaR.ensure(3 * P->memberRanks * sizeof(int));
-    flag.ensure(sizeof(int));
-    char* av = static_cast<char*>(arenaV.ptr);
-    int* ar = static_cast<int*>(arenaR.ptr);
+    // The arena stays with the program (every call on this geometry reuses
+    // it; calls run one after another on the library stream).
+    P->arenaV.ensure(3 * P->memberElems * es);
+    P->arenaR.ensure(3 * P->memberRanks * sizeof(int));
+    P->flag.ensure(sizeof(int));
+    char* av = static_cast<char*>(P->arenaV.ptr);
+    int* ar = static_cast<int*>(P->arenaR.ptr);
     HIPCHECK(hipMemcpyAsync(av + MW * P->memberElems * es, W->vals, LS->totalElems() * es, hipMemcpyDeviceToDevice, s));
     HIPCHECK(hipMemcpyAsync(ar + MW * P->memberRanks, W->ranks, LS->nLowRank() * sizeof(int), hipMemcpyDeviceToDevice, s));
-    int* sing = static_cast<int*>(flag.ptr);
+    int* sing = static_cast<int*>(P->flag.ptr);
     launch_fill_int(sing, 1, 0, s);
     if (A->prec == DMHM_F64) run_program<double>(*P, reinterpret_cast<double*>(av), ar, sing, o);
     else run_program<float>(*P, reinterpret_cast<float*>(av), ar, sing, o);
@@ -375,9 +383,6 @@ void direct_invert(dmhm_hmat_s* A, const dmhm_hh_opts& o)
     HIPCHECK(hipMemcpyAsync(R->ranks, ar + MX * P->memberRanks, LS->nLowRank() * sizeof(int), hipMemcpyDeviceToDevice, s));
     int hostSing = 0;
     HIPCHECK(copy_sync(&hostSing, sing, sizeof(int), hipMemcpyDeviceToHost));
-    arenaV.release();
-    arenaR.release();
-    flag.release();
     if (hostSing) throw Error(DMHM_ESINGULAR, "singular pivot block in the direct inversion");
     R->stats = A->stats;
     A->full = true;
