@@ -191,6 +191,79 @@ def batch_mode(a, np, dmhm_amd, W):
     print(json.dumps(out))
 
 
+def rank_mode(a, np, dmhm_amd, W):
+    """The learned-rank workload (2D 512^2, log a Gaussian random field with
+    sigma = 1.5, l = 0.05, numLevels 7, maxRank 32, FP64). Products X A in
+    tolerance mode (relative 1e-10), with left operands X = U V^T of rank r in
+    every admissible block (as the iterates of a Schulz run carry growing
+    ranks), on --rank-fields seeded coefficient fields reveal their blocks'
+    ranks; the rank model is trained on (features of X's blocks, revealed
+    rank). On a further field and a fresh X of rank 12 the full-width product
+    (sketch width maxRank + 4 everywhere) and the learned-width one (predicted
+    rank + 1 + 4; full width as the fallback where the SVD keeps more than
+    width - 4 values) are timed after their tile engines are tuned, and
+    compared."""
+    sync = dmhm_amd.load().dmhm_synchronize
+    opts = dict(oversample=4, tol_mode=1, rel_tol=1e-10)
+
+    def operator(seed):
+        cfg = W.config_operator("c4", seed=seed)
+        return cfg, dmhm_amd.HMat2d.from_csr(cfg["sizes"], cfg["num_levels"], cfg["max_rank"], True,
+                                             cfg["rowptr"], cfg["colind"], cfg["vals"])
+
+    def lowrank(cfg, r, seed):
+        n = int(np.prod(cfg["sizes"]))
+        rng = np.random.default_rng(seed)
+        return dmhm_amd.HMat2d.from_lowrank(cfg["sizes"], cfg["num_levels"], cfg["max_rank"], True,
+                                            rng.standard_normal((n, r)), rng.standard_normal((n, r)))
+
+    feats, ranks = [], []
+    for f in range(a.rank_fields):
+        cfg, A = operator(100 + f)
+        for r in (4, 10, 16, 24):
+            log("bench: field %d, left operand of rank %d: full-width product in tolerance mode" % (f, r))
+            X = lowrank(cfg, r, 1000 * f + r)
+            feats.append(X.rank_features())
+            ranks.append(X.Multiply(1.0, A, seed=f + 1, **opts).ranks().astype(np.float32))
+            del X
+        del A
+    F, t = np.concatenate(feats), np.concatenate(ranks)
+    model = dmhm_amd.RankModel(seed=7)
+    losses = model.train(F, t, steps=2000, batch=1024, lr=3e-3, seed=1)
+    log("bench: trained on %d pairs (revealed ranks 0..%d), loss %.4g -> %.4g"
+        % (len(t), int(t.max()), float(losses[:20].mean()), float(losses[-20:].mean())))
+    cfg, A = operator(100 + a.rank_fields)
+    X = lowrank(cfg, 12, 777)
+
+    def timed(**kw):
+        for _ in range(6):          # warm-up and tile-engine tuning runs, then the timed one
+            sync()
+            t0 = time.perf_counter()
+            P = X.Multiply(1.0, A, seed=99, **opts, **kw)
+            sync()
+            sec = time.perf_counter() - t0
+        return P, sec
+
+    Pf, sec_full = timed()
+    Pl, sec_learned = timed(rank_model=model, rank_margin=1.0)
+    st = dmhm_amd.last_product_stats()
+    v = np.random.default_rng(3).standard_normal(A.Height())
+    yf, yl = Pf.Multiply(1.0, v), Pl.Multiply(1.0, v)
+    diff = float(np.linalg.norm(yf - yl) / np.linalg.norm(yf))
+    log("bench: full width %.3f s, learned width %.3f s (%.2fx), sketch columns %d of %d, %d fallbacks, "
+        "relative difference %.2e" % (sec_full, sec_learned, sec_full / sec_learned, st["width_sum"],
+                                      st["full_width_sum"], st["fallbacks"], diff))
+    print(json.dumps({"mode": "rank", "config": "2D 512^2 lognormal sigma=1.5 l=0.05, numLevels 7, maxRank 32, "
+                      "FP64 X*A in tolerance mode (rel 1e-10), X of rank 12", "training_fields": a.rank_fields,
+                      "training_pairs": int(len(t)), "loss_first": float(losses[:20].mean()),
+                      "loss_last": float(losses[-20:].mean()), "full_width_s": sec_full,
+                      "learned_width_s": sec_learned, "speedup": sec_full / sec_learned,
+                      "sketch_columns": st["width_sum"], "full_width_columns": st["full_width_sum"],
+                      "fallbacks": st["fallbacks"], "blocks": st["blocks"],
+                      "relative_difference_Pv": diff,
+                      "ranks_equal": bool(np.array_equal(Pf.ranks(), Pl.ranks()))}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,10 +274,11 @@ def main():
     ap.add_argument("--switch-tol", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--mode", choices=["full", "matvec", "hh", "inverse", "batch"], default="full",
+    ap.add_argument("--mode", choices=["full", "matvec", "hh", "inverse", "batch", "rank"], default="full",
                     help="matvec / hh / inverse: only that loop (for rocprofv3 runs); inverse: the timed "
                          "inversion of the full mode, --steps times; batch: a batch of independent 3D "
                          "operators inverted together and one by one")
+    ap.add_argument("--rank-fields", type=int, default=3, help="rank mode: training coefficient fields")
     ap.add_argument("--batch-count", type=int, default=48)
     ap.add_argument("--batch-sides", default="16,20,24,28,32")
     ap.add_argument("--batch-iters", type=int, default=2, help="Schulz steps of the batch mode")
@@ -230,6 +304,9 @@ def main():
     dmhm_amd.init(0)
     if a.mode == "batch":
         batch_mode(a, np, dmhm_amd, W)
+        return
+    if a.mode == "rank":
+        rank_mode(a, np, dmhm_amd, W)
         return
     if a.config == "c4" and a.mode == "full":
         print("the 512^2 workload is measured in the kernel modes (--mode matvec / hh)", file=sys.stderr)
