@@ -568,9 +568,11 @@ def EstimateTwoNorm(A, theta=1.5, confidence=6.0, seed=1):
 
 
 class HBatch:
-    """Independent H-matrices worked on together, one launch per phase for
-    all members (dmhm_hbatch_*). No reference counterpart: the reference
-    inverts such operators one by one (src/HMat3d/Invert-incl.hpp:127-188)."""
+    """Independent H-matrices worked on together (dmhm_hbatch_*): members of
+    one class share one launch per phase, members whose products fill the
+    chip alone run one after another. No reference counterpart: the
+    reference inverts such operators one by one
+    (src/HMat3d/Invert-incl.hpp:127-188)."""
 
     def __init__(self, handle, cls_of):
         self._h = ctypes.c_void_p(handle) if not isinstance(handle, ctypes.c_void_p) else handle

@@ -311,9 +311,12 @@ int dmhm_direct_invert_program_stats(dmhm_hmat h, int oversample, long long* pro
  * inverts such operators one after the other, HMat3d::SchulzInvert per
  * operator, src/HMat3d/Invert-incl.hpp:127-188). Members may differ in
  * dimension, grid, levels and rank but share a precision; each batch
- * operation runs every member's work in one launch per phase. A member's
- * result is the result of the same operation on that member alone (same
- * seeds), up to the tile engine each merged phase is tuned to. */
+ * operation runs the members of one class (same geometry, layouts and
+ * operand forms) in one launch per phase, chunked by the workspace budget,
+ * while members whose products fill the chip alone (a tile phase of 65,536
+ * tiles or more) run one after another. A member's result is the result of
+ * the same operation on that member alone (same seeds), up to the tile engine
+ * each phase is tuned to. */
 int dmhm_hbatch_create(int count, const dmhm_hmat* members, dmhm_hbatch* out);   /* copies the members */
 int dmhm_hbatch_destroy(dmhm_hbatch b);
 int dmhm_hbatch_count(dmhm_hbatch b, int* count);
