@@ -78,6 +78,31 @@ def test_dense_leaf_inverse_with_pivoting(sizes):
         assert np.abs(X - Xr).max() < tol * np.abs(Xr).max(), (prec, np.abs(X - Xr).max() / np.abs(Xr).max())
 
 
+def test_direct_invert_repeated_calls_are_identical():
+    """The direct-inversion program and its arena stay with the geometry
+    between calls: with one tile engine forced, every later call's result
+    equals the first one bit for bit (same kernels and seeds), also when calls
+    on another geometry interleave."""
+    op, N, args = _op(16, 3, 2, smooth=False, seed=5)
+    op2, N2, args2 = _op(12, 3, 2, smooth=False, seed=6)
+    A = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 8, True, *args)
+    B = dmhm_amd.HMat2d.from_csr(op2["sizes"], 3, 8, True, *args2)
+    prev = dmhm_amd.set_tile_engine("lds")     # one engine: tuning would pick per run
+    try:
+        ref = A.Copy().DirectInvert().to_dense()
+        refB = B.Copy().DirectInvert().to_dense()
+        for i in range(6):
+            print("call", i, flush=True)
+            X = A.Copy().DirectInvert().to_dense()
+            assert np.array_equal(X, ref), i
+            Y = B.Copy().DirectInvert().to_dense()
+            assert np.array_equal(Y, refB), i
+    finally:
+        dmhm_amd.set_tile_engine(prev)
+    D = W.csr_to_dense(*args, N)
+    assert np.linalg.norm(np.eye(N) - ref @ D, 2) < 1e-2
+
+
 def test_direct_invert_fp32():
     op, N, args = _op(16, 3, 2)
     A = dmhm_amd.HMat2d.from_csr(op["sizes"], 3, 16, True, *args, precision=dmhm_amd.DMHM_F32)
