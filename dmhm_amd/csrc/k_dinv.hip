@@ -155,113 +155,114 @@ __global__ __launch_bounds__(NT) void dense_inverse_kernel(const T* __restrict__
         for (int e = tid; e < mm; e += NT) dst[e] = a[e];
 }
 
-// The same inversion for m <= 64 with the matrix in registers, by 64/RW
-// waves: lane c holds column c, wave g rows RW*g .. RW*g+RW-1. Step k: every
-// lane takes the largest magnitude of its own column over its rows >= k and
-// the wave's candidate is lane k's (readlane); the waves' candidates meet in
-// LDS and every thread picks the same p (largest magnitude, lowest row on
-// ties); the owners of rows p and k publish them in LDS, and each thread
-// updates its entries from the published pivot row, reading the multipliers
-// (column k) from lane k and exchanging rows k and p on the fly. Two barriers
-// per step; more waves with fewer rows each hide the latency of the step's
-// dependent arithmetic. The row exchanges are undone as column exchanges by
-// composing them into one column permutation (sigma(c) = s_m-1(...s_0(c)),
-// s_k swapping k and p_k), applied while the result is staged through LDS for
-// a coalesced store.
-template <typename T, int RW>
-__global__ void __launch_bounds__(64 * (64 / RW)) dense_inverse_reg_kernel(const T* __restrict__ src,
-                                                                          T* __restrict__ dst, int m,
-                                                                          int* __restrict__ singular)
+// Largest unsigned value over the wave (DPP within rows of 16, then the four
+// row maxima through scalar registers); the result is uniform.
+__device__ inline uint32_t wave_max_u32(uint32_t x)
 {
-    constexpr int NWV = 64 / RW, LD = 65;
-    __shared__ double tile[64 * LD];          // tile[j * LD + i] = a(i, j)
-    __shared__ double prow[64], krow[64];
-    __shared__ double cval[NWV];
-    __shared__ int cidx[NWV];
-    __shared__ int sigma[64];
-    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6, r0 = g * RW;
-    for (int j = g; j < 64; j += NWV)
-        tile[j * LD + c] = (j < m && c < m) ? (double)src[c + (int64_t)j * m] : 0.0;
-    __syncthreads();
-    double a[RW];
-#pragma unroll
-    for (int r = 0; r < RW; ++r) a[r] = tile[c * LD + r0 + r];
-    int x = c;                                // source column of result column c
-#pragma unroll 1
-    for (int k = 0; k < m; ++k) {
-        double best = -1.0;
-        int bi = 0;
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int i = r0 + r;
-            const double v = (i >= k && i < m) ? fabs(a[r]) : -1.0;
-            if (v > best) { best = v; bi = i; }
-        }
-        best = bcast(best, k);
-        bi = __builtin_amdgcn_readlane(bi, k);
-        if (c == 0) { cval[g] = best; cidx[g] = bi; }
-        __syncthreads();
-        double pb = cval[0];
-        int p = cidx[0];
-#pragma unroll
-        for (int q = 1; q < NWV; ++q)
-            if (cval[q] > pb) { pb = cval[q]; p = cidx[q]; }
-        p = __builtin_amdgcn_readfirstlane(p);
-        const int gk = k / RW, rk = k % RW, gp = p / RW, rp = p % RW;
-        if (g == gp) {
-            double v = 0.0;
-#pragma unroll
-            for (int r = 0; r < RW; ++r) v = (r == rp) ? a[r] : v;
-            prow[c] = v;
-        }
-        if (g == gk) {
-            double v = 0.0;
-#pragma unroll
-            for (int r = 0; r < RW; ++r) v = (r == rk) ? a[r] : v;
-            krow[c] = v;
-        }
-        __syncthreads();
-        const double d = prow[k];             // the pivot
-        const double fp = krow[k];            // row p's new entry in column k (old a(k,k))
-        const double oldk = krow[c];          // row p's new entry in column c
-        if (d == 0.0 && tid == 0) atomicExch(singular, 1);
-        const double inv = d == 0.0 ? 0.0 : 1.0 / d;
-        const double rkc = prow[c] * inv;     // pivot row, scaled
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int i = r0 + r;
-            const double ck = bcast(a[r], k);  // this row's entry in column k
-            const bool swapped = (i == p) && (p != k);
-            const double f = swapped ? fp : ck;
-            const double cur = swapped ? oldk : a[r];
-            double v;
-            if (i == k) v = (c == k) ? inv : rkc;
-            else v = (c == k) ? -f * inv : cur - f * rkc;
-            a[r] = v;
-        }
-        x = (x == k) ? p : ((x == p) ? k : x);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RW; ++r) tile[c * LD + r0 + r] = a[r];
-    sigma[c] = x;
-    __syncthreads();
-    for (int j = g; j < m; j += NWV)
-        if (c < m) dst[c + (int64_t)j * m] = (T)tile[sigma[j] * LD + c];
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppXor1, 0xF, 0xF, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppXor2, 0xF, 0xF, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppMirror8, 0xF, 0xF, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppMirror16, 0xF, 0xF, false));
+    const uint32_t a = __builtin_amdgcn_readlane(x, 0), b = __builtin_amdgcn_readlane(x, 16);
+    const uint32_t c = __builtin_amdgcn_readlane(x, 32), e = __builtin_amdgcn_readlane(x, 48);
+    return max(max(a, b), max(c, e));
 }
 
-// Rows per wave of the register inversion (8 waves of 8 rows: 108 us for a
-// 64 x 64 leaf against 122 us with 4 waves, 166 us with 16 and 178 us for
-// the LDS kernel with 512 threads; publishing the candidate rows to save the
-// second barrier gained 3%).
-constexpr int kInvRows = 8;
+// Inverse of one m x m leaf (m <= 64): Gauss-Jordan with partial pivoting,
+// lane = row, the 64 columns split over NWV waves (wave w holds columns
+// [w CW, (w + 1) CW) of every row in registers). Three departures from the
+// textbook form keep each step to one barrier and one LDS exchange:
+// - rows are not exchanged: step k takes the largest |u(i,k)| over the rows
+//   not yet used as pivots (compared on the high word of the double: sign
+//   cleared, exponent and 20 mantissa bits, ties to the lowest row) and
+//   records the pivot row p_k. The elimination then leaves
+//   inv(A)(k, p_j) = T(p_k, j) for the final array T, applied at the store;
+// - the pivot row is not scaled by 1/u(p,k) at its step: the factor of every
+//   other row is u(i,k) / u(p,k) either way, so the scale is kept per row and
+//   applied at the store (the row's entries stay, its column-k entry is 1);
+// - the wave that owns column k chooses the pivot and publishes the pivot
+//   row's index, 1/u(p,k) and every row's factor through LDS; every wave then
+//   updates only its own columns, reading its share of the pivot row from its
+//   own lane p (v_readlane). Buffers alternate by step parity, so a step's
+//   writes never meet the previous step's reads.
+// Padding rows and columns beyond m hold the identity; their steps (up to
+// CW - 1 in the last panel) change nothing.
+template <typename T, int NWV>
+__global__ void __launch_bounds__(64 * NWV) dense_inverse_split_kernel(const T* __restrict__ src,
+                                                                      T* __restrict__ dst, int m,
+                                                                      int* __restrict__ singular)
+{
+    constexpr int CW = 64 / NWV;
+    static_assert(CW % 2 == 0, "step parity follows the column within a wave");
+    __shared__ double gbuf[2][64], ibuf[2];
+    __shared__ int pbuf[2];
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double u[CW];
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+        const int c = w * CW + j;
+        u[j] = (c < m && lane < m) ? (double)src[lane + (int64_t)c * m] : (c == lane ? 1.0 : 0.0);
+    }
+    bool used = false, sing = false;
+    double scale = 1.0;
+    int stepOf = 0, pivOf = 0;      // this row's step; the pivot row of step `lane`
+    const int panels = (m + CW - 1) / CW;
+#pragma unroll 1
+    for (int kb = 0; kb < panels; ++kb) {
+        const bool owner = w == kb;
+#pragma unroll
+        for (int kk = 0; kk < CW; ++kk) {
+            const int k = kb * CW + kk, par = kk & 1;
+            if (owner) {
+                const double v = u[kk];
+                const uint32_t hi = used ? 0u : ((uint32_t)__double2hiint(v) & 0x7fffffffu);
+                const uint32_t mx = wave_max_u32(hi);
+                const unsigned long long cand = __ballot(!used && hi == mx);
+                const int p = __builtin_amdgcn_readfirstlane(__ffsll(cand) - 1);
+                const double dd = bcast(v, p);
+                const double inv = dd == 0.0 ? 0.0 : 1.0 / dd;
+                gbuf[par][lane] = lane == p ? 0.0 : v * inv;
+                if (lane == 0) { pbuf[par] = p; ibuf[par] = inv; }
+                sing = sing || dd == 0.0;
+            }
+            __syncthreads();
+            const int p = __builtin_amdgcn_readfirstlane(pbuf[par]);
+            const double inv = ibuf[par];
+            const double g = gbuf[par][lane];
+            const bool me = lane == p;
+#pragma unroll
+            for (int j = 0; j < CW; ++j) {
+                if (j == kk && owner) u[j] = me ? 1.0 : -g;
+                else u[j] = fma(-g, bcast(u[j], p), u[j]);
+            }
+            scale = me ? inv : scale;
+            used = used || me;
+            stepOf = me ? k : stepOf;
+            pivOf = lane == k ? p : pivOf;
+        }
+    }
+    if (sing && lane == 0) atomicExch(singular, 1);
+    const bool row = lane < m && stepOf < m;
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+        const int c = w * CW + j;
+        const int pc = __builtin_amdgcn_readlane(pivOf, c);
+        if (row && c < m && pc < m) dst[stepOf + (int64_t)pc * m] = (T)(u[j] * scale);
+    }
+}
+
+// Waves of the leaf inversion (a 64 x 64 FP64 leaf: 28 us with 8 waves or
+// 16, 32 us with 4, 66 us with one wave that broadcasts the whole pivot row
+// by lane reads, against 108 us for lane = column with rows split over 8
+// waves and two barriers per step).
+constexpr int kInvWaves = 8;
 
 template <typename T>
 void launch_dense_inverse(const T* src, T* dst, int m, int* singular, hipStream_t s)
 {
     if (m <= 0) return;
     if (m <= 64) {
-        dense_inverse_reg_kernel<T, kInvRows><<<1, 64 * (64 / kInvRows), 0, s>>>(src, dst, m, singular);
+        dense_inverse_split_kernel<T, kInvWaves><<<1, 64 * kInvWaves, 0, s>>>(src, dst, m, singular);
         HIP_LAUNCH_CHECK();
         return;
     }
