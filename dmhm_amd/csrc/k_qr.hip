@@ -123,316 +123,194 @@ __global__ void __launch_bounds__(256) qr_kernel(Bases b, const QrTask* __restri
     }
 }
 
-// Register-resident Householder QR for panels of m <= 64*NW rows and
-// k <= KMAX columns: one workgroup of NW waves per panel, thread t holding
-// row t of the panel in registers. The reflector loop runs at run time; the
-// row's registers stay statically indexed because column j is read and
-// written through unrolled compare-selects over the KMAX columns. Each
-// reflector step needs one reduction round: the squared norm of the column
-// below the diagonal and its products with the later columns are reduced
-// together (wave sums, then the NW wave partials through LDS), and the pivot
-// row is published through LDS in the same round, so a step costs one
-// barrier (the LDS buffers alternate between steps). Q is formed in place
-// (orgqr order). FP64 arithmetic; same reflector convention and outputs as
-// qr_kernel; the panel and R may be strided (leading dimensions ld, rld), as
-// the row blocks of a TSQR are.
-template <int KMAX>
-__device__ __forceinline__ double pick_col(const double (&a)[KMAX], int j)
+// Wave sum with a uniform result: DPP sums inside each row of 16 lanes, then
+// the four row sums through scalar registers (no LDS round trip).
+__device__ inline double wave_sum_uniform(double v)
 {
-    double v = 0.0;
-#pragma unroll
-    for (int c = 0; c < KMAX; ++c) v = (c == j) ? a[c] : v;
-    return v;
+    v = group_sum<16>(v);
+    return (bcast(v, 0) + bcast(v, 16)) + (bcast(v, 32) + bcast(v, 48));
 }
 
-template <typename T, int KMAX, int NW, bool BATCHED>
-__global__ void __launch_bounds__(64 * NW) qr_reg_kernel(Bases b, const QrTask* __restrict__ tasks)
+// Householder QR of one m x k panel (m <= 64 RR, k <= KMAX), lane = row (rows
+// lane + 64 r), the KMAX columns split over NWV waves (wave w holds columns
+// [w CW, (w + 1) CW) of every row in registers). Step j: the wave that owns
+// column j forms the reflector (its column's sum of squares below the
+// diagonal is one wave reduction), publishes v and tau through LDS, and
+// every wave then applies it to its own columns right of j (one wave
+// reduction v^T a_c per column). One barrier per step; buffers alternate by
+// step parity, so a step's writes never meet the reads of the step before.
+// The explicit Q is accumulated backwards the same way (orgqr order). Same
+// reflector convention and outputs as qr_kernel (Q over the panel, R to the
+// workspace); the panel and R may be strided (leading dimensions ld, rld), as
+// the row blocks of a TSQR are. FP64 arithmetic.
+template <typename T, int KMAX, int NWV, int RR>
+__global__ void __launch_bounds__(64 * NWV) qr_split_kernel(Bases b, const QrTask* __restrict__ tasks)
 {
-    __shared__ double part[2][NW][KMAX];
-    __shared__ double rowj[2][KMAX];
+    constexpr int CW = KMAX / NWV;
+    static_assert(KMAX % NWV == 0, "whole columns per wave");
+    __shared__ double vbuf[2][64 * RR];
+    __shared__ double tbuf[2];
     __shared__ double taus[KMAX];
     const QrTask tk = tasks[blockIdx.x];
     T* A = reinterpret_cast<T*>(b.val[tk.slot]) + tk.off;
     const int m = tk.m, kstat = tk.k, ld = tk.ld;
-    // Effective width (width-indirect sketches): columns >= k are left alone.
     const int k = tk.krank >= 0 ? min(kstat, b.rank[tk.kslot][tk.krank]) : kstat, kk = min(m, k);
     if (k <= 0) return;
-    const int i = threadIdx.x, w = i >> 6, lane = i & 63;
-    const bool inPanel = i < m;
-    // BATCHED: all column sums of a step reduced together (lowest latency,
-    // more registers: occupancy 2), for phases of few panels; else one
-    // column at a time (more panels in flight), for large phases.
-    constexpr bool kBatchedSums = BATCHED;
-    double a[KMAX];
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double a[CW][RR];
 #pragma unroll
-    for (int c = 0; c < KMAX; ++c) a[c] = (inPanel && c < k) ? (double)A[i + (int64_t)c * ld] : 0.0;
-
-#pragma unroll 1
-    for (int j = 0; j < kk; ++j) {
-        const int buf = j & 1;
-        const bool below = i > j && inPanel;
-        const double aj = pick_col(a, j);
-        if (i == j) {
+    for (int jj = 0; jj < CW; ++jj)
 #pragma unroll
-            for (int c = 0; c < KMAX; ++c)
-                if (c >= j) rowj[buf][c] = a[c];
+        for (int r = 0; r < RR; ++r) {
+            const int c = w * CW + jj, i = lane + 64 * r;
+            a[jj][r] = (i < m && c < k) ? (double)A[i + (int64_t)c * ld] : 0.0;
         }
-        // Column products (zero outside columns j..k-1): reduced together
-        // where the registers allow it (latency paid once per butterfly
-        // step), else one column at a time.
-        if constexpr (kBatchedSums) {
-            double sv[KMAX];
+    // Applies the published reflector (v, tau) of step j to this wave's columns c, j < c < k.
+    auto apply = [&](int j, int par) {
+        if (w * CW + CW - 1 <= j || w * CW >= k) return;       // no column of this wave is right of j
+        const double t = tbuf[par];
+        double v[RR];
 #pragma unroll
-            for (int c = 0; c < KMAX; ++c) sv[c] = (below && c >= j && c < k) ? aj * a[c] : 0.0;
-            wave_sum_all<KMAX>(sv);
-            if (lane == 0) {
+        for (int r = 0; r < RR; ++r) v[r] = vbuf[par][lane + 64 * r];
+        double dot[CW];
 #pragma unroll
-                for (int c = 0; c < KMAX; ++c) part[buf][w][c] = sv[c];
+        for (int jj = 0; jj < CW; ++jj) {
+            double s = 0.0;
+#pragma unroll
+            for (int r = 0; r < RR; ++r) s += v[r] * a[jj][r];
+            dot[jj] = wave_sum_uniform(s);
+        }
+#pragma unroll
+        for (int jj = 0; jj < CW; ++jj) {
+            const int c = w * CW + jj;
+            if (c > j && c < k) {
+                const double f = t * dot[jj];
+#pragma unroll
+                for (int r = 0; r < RR; ++r) a[jj][r] -= f * v[r];
             }
-        } else {
+        }
+    };
+    int step = 0;
+#pragma unroll 1
+    for (int j = 0; j < kk; ++j, ++step) {
+        const int par = step & 1;
+        if (w == j / CW) {
+            const int js = j - w * CW;
+            double x[RR];
 #pragma unroll
-            for (int c = 0; c < KMAX; ++c) {
-                if (c >= j && c < k) {
-                    const double sum = wave_sum(below ? aj * a[c] : 0.0);
-                    if (lane == 0) part[buf][w][c] = sum;
-                }
+            for (int r = 0; r < RR; ++r) {
+                x[r] = 0.0;
+#pragma unroll
+                for (int jj = 0; jj < CW; ++jj) x[r] = jj == js ? a[jj][r] : x[r];
             }
+            double s = 0.0, xj = 0.0;
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const int i = lane + 64 * r;
+                const double xb = (i > j && i < m) ? x[r] : 0.0;
+                s += xb * xb;
+                xj = r == (j >> 6) ? x[r] : xj;
+            }
+            const double xn2 = wave_sum_uniform(s);
+            const double alpha = bcast(xj, j & 63);
+            double t = 0.0, scale = 0.0, beta = alpha;
+            if (xn2 > 0.0 && alpha * alpha + xn2 > kTinyColumn) {
+                beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                t = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const int i = lane + 64 * r;
+                const bool below = i > j && i < m;
+                vbuf[par][i] = below ? x[r] * scale : (i == j ? 1.0 : 0.0);
+                const double nx = i == j ? beta : (i > j ? x[r] * scale : x[r]);
+#pragma unroll
+                for (int jj = 0; jj < CW; ++jj) a[jj][r] = jj == js ? nx : a[jj][r];
+            }
+            if (lane == 0) { tbuf[par] = t; taus[j] = t; }
         }
         __syncthreads();
-        double xn2 = 0.0;
-#pragma unroll
-        for (int q = 0; q < NW; ++q) xn2 += part[buf][q][j];
-        const double alpha = rowj[buf][j];
-        double t = 0.0, scale = 0.0, beta = alpha;
-        if (xn2 > 0.0 && alpha * alpha + xn2 > kTinyColumn) {
-            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-            t = (beta - alpha) / beta;
-            scale = 1.0 / (alpha - beta);
-        }
-        const double vi = aj * scale;          // reflector entry of this row (rows below j)
-#pragma unroll
-        for (int c = 0; c < KMAX; ++c) {
-            if (c > j && c < k) {
-                double d = 0.0;
-#pragma unroll
-                for (int q = 0; q < NW; ++q) d += part[buf][q][c];
-                const double wc = t * (rowj[buf][c] + scale * d);
-                if (i == j) a[c] -= wc;
-                else if (below) a[c] -= wc * vi;
-            }
-        }
-        const double newj = (i == j) ? beta : (below ? vi : aj);
-#pragma unroll
-        for (int c = 0; c < KMAX; ++c)
-            if (c == j) a[c] = newj;
-        if (i == 0) taus[j] = t;
+        apply(j, par);
     }
-    __syncthreads();
-    // R: rows 0..kstat-1 of the first k columns (rows >= kk zero, so stacked R
-    // factors of a TSQR carry zero rows below a narrower effective width).
-    if (tk.roff >= 0 && i < kstat) {
+    if (tk.roff >= 0) {
         T* R = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.roff;
 #pragma unroll
-        for (int c = 0; c < KMAX; ++c)
-            if (c < k) R[i + (int64_t)c * tk.rld] = (i <= c && i < kk) ? (T)a[c] : T(0);
+        for (int jj = 0; jj < CW; ++jj)
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const int c = w * CW + jj, i = lane + 64 * r;
+                if (c < k && i < kstat) R[i + (int64_t)c * tk.rld] = (i <= c && i < kk) ? (T)a[jj][r] : T(0);
+            }
     }
-    // Q in place: columns >= kk start at zero, then H_j for j = kk-1 .. 0.
 #pragma unroll
-    for (int c = 0; c < KMAX; ++c)
-        if (c >= kk) a[c] = 0.0;
+    for (int jj = 0; jj < CW; ++jj)
+        if (w * CW + jj >= kk) {
+#pragma unroll
+            for (int r = 0; r < RR; ++r) a[jj][r] = 0.0;
+        }
 #pragma unroll 1
-    for (int j = kk - 1; j >= 0; --j) {
-        const int buf = j & 1;
-        const double t = taus[j];
-        const double aj = pick_col(a, j);
-        const double v = (i == j) ? 1.0 : ((i > j && inPanel) ? aj : 0.0);
-        if constexpr (kBatchedSums) {
-            double sv[KMAX];
+    for (int j = kk - 1; j >= 0; --j, ++step) {
+        const int par = step & 1;
+        if (w == j / CW) {
+            const int js = j - w * CW;
+            const double t = taus[j];
 #pragma unroll
-            for (int c = 0; c < KMAX; ++c) sv[c] = (c > j && c < k) ? v * a[c] : 0.0;
-            wave_sum_all<KMAX>(sv);
-            if (lane == 0) {
+            for (int r = 0; r < RR; ++r) {
+                const int i = lane + 64 * r;
+                double x = 0.0;
 #pragma unroll
-                for (int c = 0; c < KMAX; ++c) part[buf][w][c] = sv[c];
+                for (int jj = 0; jj < CW; ++jj) x = jj == js ? a[jj][r] : x;
+                const double v = i == j ? 1.0 : ((i > j && i < m) ? x : 0.0);
+                vbuf[par][i] = v;
+                const double nx = i == j ? 1.0 - t : (i > j ? -t * v : 0.0);
+#pragma unroll
+                for (int jj = 0; jj < CW; ++jj) a[jj][r] = jj == js ? nx : a[jj][r];
             }
-        } else {
-#pragma unroll
-            for (int c = 0; c < KMAX; ++c) {
-                if (c > j && c < k) {
-                    const double sum = wave_sum(v * a[c]);
-                    if (lane == 0) part[buf][w][c] = sum;
-                }
-            }
+            if (lane == 0) tbuf[par] = t;
         }
         __syncthreads();
-#pragma unroll
-        for (int c = 0; c < KMAX; ++c) {
-            if (c > j && c < k) {
-                double d = 0.0;
-#pragma unroll
-                for (int q = 0; q < NW; ++q) d += part[buf][q][c];
-                a[c] -= t * d * v;
-            }
-        }
-        const double newj = (i == j) ? 1.0 - t : ((i > j && inPanel) ? -t * aj : 0.0);
-#pragma unroll
-        for (int c = 0; c < KMAX; ++c)
-            if (c == j) a[c] = newj;
+        apply(j, par);
     }
-    if (inPanel) {
 #pragma unroll
-        for (int c = 0; c < KMAX; ++c)
-            if (c < k) A[i + (int64_t)c * ld] = (T)a[c];
-    }
-}
-
-// Householder QR of a panel of m <= NW*RW rows and k <= 64 columns by NW
-// waves with the panel in registers, transposed to the other kernels: lane c
-// holds column c, wave w rows RW*w .. RW*w+RW-1. Step j: each lane reads the
-// reflector column j from lane j of its wave (readlane, uniform) and forms
-// its column's product with it over the wave's rows; the NW wave partials
-// (column j's own product is the squared norm below the diagonal) are summed
-// through LDS, where the wave holding row j also publishes that row. One
-// barrier per step (the LDS buffers alternate) and no cross-lane reduction.
-// Q is formed in place in orgqr order the same way. Same reflector
-// convention, R and Q as qr_kernel.
-template <typename T, int RW, int NW>
-__global__ void __launch_bounds__(64 * NW) qr_col_kernel(Bases b, const QrTask* __restrict__ tasks)
-{
-    __shared__ double part[2][NW][64];
-    __shared__ double rowj[2][64];
-    __shared__ double taus[64];
-    const QrTask tk = tasks[blockIdx.x];
-    T* A = reinterpret_cast<T*>(b.val[tk.slot]) + tk.off;
-    const int m = tk.m, kstat = tk.k, ld = tk.ld;
-    const int k = tk.krank >= 0 ? min(kstat, b.rank[tk.kslot][tk.krank]) : kstat, kk = min(m, k);
-    if (k <= 0) return;
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6, r0 = w * RW;
-    double a[RW];
+    for (int jj = 0; jj < CW; ++jj)
 #pragma unroll
-    for (int r = 0; r < RW; ++r) a[r] = (c < k && r0 + r < m) ? (double)A[r0 + r + (int64_t)c * ld] : 0.0;
-#pragma unroll 1
-    for (int j = 0; j < kk; ++j) {
-        const int buf = j & 1;
-        double dot = 0.0;
-#pragma unroll
-        for (int r = 0; r < RW; ++r)
-            if (r0 + r > j) dot += bcast(a[r], j) * a[r];
-        part[buf][w][c] = dot;
-        if (w == j / RW) {
-            double v = 0.0;
-#pragma unroll
-            for (int r = 0; r < RW; ++r) v = (r0 + r == j) ? a[r] : v;
-            rowj[buf][c] = v;
+        for (int r = 0; r < RR; ++r) {
+            const int c = w * CW + jj, i = lane + 64 * r;
+            if (c < k && i < m) A[i + (int64_t)c * ld] = (T)a[jj][r];
         }
-        __syncthreads();
-        double xn2 = 0.0, dc = 0.0;
-#pragma unroll
-        for (int q = 0; q < NW; ++q) { xn2 += part[buf][q][j]; dc += part[buf][q][c]; }
-        const double alpha = rowj[buf][j];
-        double t = 0.0, scale = 0.0, beta = alpha;
-        if (xn2 > 0.0 && alpha * alpha + xn2 > kTinyColumn) {
-            beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-            t = (beta - alpha) / beta;
-            scale = 1.0 / (alpha - beta);
-        }
-        const double wc = t * (rowj[buf][c] + scale * dc);
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int i = r0 + r;
-            const double vi = bcast(a[r], j) * scale;     // reflector entry of row i (rows below j)
-            if (c == j) a[r] = (i == j) ? beta : ((i > j) ? vi : a[r]);
-            else if (c > j && i == j) a[r] -= wc;
-            else if (c > j && i > j) a[r] -= wc * vi;
-        }
-        if (threadIdx.x == 0) taus[j] = t;
-    }
-    __syncthreads();
-    if (tk.roff >= 0 && c < k) {
-        T* R = reinterpret_cast<T*>(b.val[SLOT_WS]) + tk.roff;
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int i = r0 + r;
-            if (i < kstat) R[i + (int64_t)c * tk.rld] = (i <= c && i < kk) ? (T)a[r] : T(0);
-        }
-    }
-    if (c >= kk) {
-#pragma unroll
-        for (int r = 0; r < RW; ++r) a[r] = 0.0;
-    }
-#pragma unroll 1
-    for (int j = kk - 1; j >= 0; --j) {
-        const int buf = j & 1;
-        const double t = taus[j];
-        double dot = 0.0;
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int i = r0 + r;
-            const double v = (i == j) ? 1.0 : ((i > j) ? bcast(a[r], j) : 0.0);
-            dot += v * a[r];
-        }
-        part[buf][w][c] = dot;
-        __syncthreads();
-        double dc = 0.0;
-#pragma unroll
-        for (int q = 0; q < NW; ++q) dc += part[buf][q][c];
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int i = r0 + r;
-            const double v = (i == j) ? 1.0 : ((i > j) ? bcast(a[r], j) : 0.0);
-            if (c == j) a[r] = (i == j) ? 1.0 - t : ((i > j) ? -t * v : 0.0);
-            else if (c > j && c < k) a[r] -= t * dc * v;
-        }
-    }
-    if (c < k) {
-#pragma unroll
-        for (int r = 0; r < RW; ++r)
-            if (r0 + r < m) A[r0 + r + (int64_t)c * ld] = (T)a[r];
-    }
 }
 
 // Panel classes of a QR phase (tasks are sorted by class at upload):
-// 0: m <= 64, 1: m <= 256, 2: m <= 512 (k <= 40), 3: m <= 1024 with k <= 24
-// (16 waves of at most 128 registers fill a CU), 4: the rest (qr_kernel).
-// Latency form (batched column sums) for classes of fewer panels than this:
-// they fit the chip with room to spare, so the time is one panel's latency.
-constexpr int kQrLatencyPanels = 512;
-
-template <typename T, int KMAX, int NW>
-static void launch_qr_reg(int n, const QrTask* t, const Bases& b, hipStream_t s)
-{
-    if constexpr (KMAX == 24 && NW <= 8) {
-        if (n < kQrLatencyPanels) {
-            qr_reg_kernel<T, KMAX, NW, true><<<n, 64 * NW, 0, s>>>(b, t);
-            return;
-        }
-    }
-    qr_reg_kernel<T, KMAX, NW, false><<<n, 64 * NW, 0, s>>>(b, t);
-}
+// 0: m <= 64, 1: m <= 256, 2: m <= 512 (k <= 40), 3: m <= 1024 with k <= 24,
+// 4: the rest (qr_kernel). Classes 0-3 take the column-split kernel with
+// 64, 256, 512 and 1024 rows in registers. Measured (FP64, k = 20, one launch):
+// 80 panels of 64 rows 21 us (64 us with lane = column and rows over 8 waves,
+// 162 us with one wave per panel), 2,000 of them 93 us with 8 waves (103 us
+// with 12, 193 us with one wave), 80 panels of 256 rows 28 us (166 us with
+// 4 waves of lane = row and the column sums through LDS), 100 of 1024 rows
+// 72 us (374 us).
+constexpr int kQrManyPanels = 512;
 
 template <typename T, int KMAX>
 static void launch_qr_classes(const QrTask* tasks, const int* classEnd, const Bases& b, hipStream_t s)
 {
+    constexpr int NWV = KMAX == 24 ? 12 : 10;      // 2 or 4 columns per wave
     int begin = 0;
     for (int c = 0; c < 4; ++c) {
         const int n = classEnd[c] - begin;
         if (n > 0) {
             const QrTask* t = tasks + begin;
-            // Measured per class (FP64, k = 20 and 36, 80 and 2,000 panels):
-            // few panels (latency) of <= 64 rows and few wide panels of <= 256
-            // rows take the column-per-lane kernel; otherwise the row-per-lane
-            // kernel (with batched column sums for few 24-column panels),
-            // except for many wide panels of <= 64 rows.
-            const bool few = n < kQrLatencyPanels;
             if (c == 0) {
-                if (few || KMAX > 24) qr_col_kernel<T, 8, 8><<<n, 512, 0, s>>>(b, t);
-                else qr_reg_kernel<T, KMAX, 1, false><<<n, 64, 0, s>>>(b, t);
+                if (KMAX == 24 && n >= kQrManyPanels) qr_split_kernel<T, 24, 8, 1><<<n, 512, 0, s>>>(b, t);
+                else qr_split_kernel<T, KMAX, NWV, 1><<<n, 64 * NWV, 0, s>>>(b, t);
             } else if (c == 1) {
-                if (few && KMAX > 24) qr_col_kernel<T, 16, 16><<<n, 1024, 0, s>>>(b, t);
-                else launch_qr_reg<T, KMAX, 4>(n, t, b, s);
+                qr_split_kernel<T, KMAX, NWV, 4><<<n, 64 * NWV, 0, s>>>(b, t);
+            } else if (c == 2) {
+                qr_split_kernel<T, KMAX, NWV, 8><<<n, 64 * NWV, 0, s>>>(b, t);
+            } else {
+                qr_split_kernel<T, 24, 12, 16><<<n, 768, 0, s>>>(b, t);    // class 3 has k <= 24
             }
-            else if (c == 2) launch_qr_reg<T, KMAX, 8>(n, t, b, s);
-            else launch_qr_reg<T, 24, 16>(n, t, b, s);    // class 3 has k <= 24
         }
         begin = classEnd[c];
     }

@@ -69,23 +69,6 @@ __device__ inline float wave_sum(float v)
     v += __shfl_xor(v, 32, 64);
     return v;
 }
-// Wave sums of N values at once: the DPP steps of all values, then each
-// cross-row step issues the N shuffles together, so their latency is paid
-// once per step instead of once per value.
-template <int N>
-__device__ __forceinline__ void wave_sum_all(double (&v)[N])
-{
-#pragma unroll
-    for (int c = 0; c < N; ++c) v[c] = group_sum<16>(v[c]);
-#pragma unroll
-    for (int off = 16; off <= 32; off <<= 1) {
-        double t[N];
-#pragma unroll
-        for (int c = 0; c < N; ++c) t[c] = __shfl_xor(v[c], off, 64);
-#pragma unroll
-        for (int c = 0; c < N; ++c) v[c] += t[c];
-    }
-}
 // Broadcast lane `l` (wave-uniform) of v to the whole wave.
 __device__ inline float bcast(float v, int l)
 {
