@@ -15,6 +15,40 @@ namespace dmhm {
 // ---------------------------------------------------------------------------
 // KMAX bounds k (LDS sized for it); NT threads, TPP (a power of two <= 8)
 // per column pair.
+// Rotation (c, s) of a column pair with squared norms al, be and inner
+// product ga that makes the pair orthogonal: t = tan(theta) is the root of
+// t^2 + 2 zeta t - 1 = 0 of smaller magnitude, zeta = (be - al) / (2 ga),
+// written as t = sgn(d) g / (|d| + sqrt(d^2 + g^2)) with d = be - al and
+// g = 2 ga. t does not change when d and g are scaled together, so both are
+// scaled by an estimate of 1 / max(|d|, |g|) first (no overflow or underflow
+// in the squares). c = 1 / sqrt(1 + t^2), s = c t. The reciprocals and
+// reciprocal square roots are the hardware estimates (v_rcp_f64, v_rsq_f64)
+// refined by two Newton steps each: a shorter dependent chain than IEEE
+// division and square root, and c^2 + s^2 = 1 to rounding either way.
+__device__ inline double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ inline double rsq_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+}
+__device__ inline void jacobi_rotation(double al, double be, double ga, double& cs, double& sn)
+{
+    const double d0 = be - al, g0 = 2.0 * ga;
+    const double rm = __builtin_amdgcn_rcp(fmax(fabs(d0), fabs(g0)));
+    const double dd = d0 * rm, gg = g0 * rm;
+    const double h = fma(dd, dd, gg * gg);
+    const double den = fabs(dd) + h * rsq_nr(h);
+    const double t = (dd >= 0.0 ? gg : -gg) * rcp_nr(den);
+    cs = rsq_nr(fma(t, t, 1.0));
+    sn = cs * t;
+}
+
 template <typename T, int KMAX, int NT, int TPP = 8>
 __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restrict__ tasks,
                                                  int tolMode, double relTol)
@@ -74,10 +108,9 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
                 al = group_sum<TPP>(al);
                 be = group_sum<TPP>(be);
                 ga = group_sum<TPP>(ga);
-                if (q < k && fabs(ga) > 1e-15 * sqrt(al * be) && fabs(ga) > 1e-30 * fro2) {
-                    const double zeta = (be - al) / (2.0 * ga);
-                    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                if (q < k && ga * ga > 1e-30 * (al * be) && fabs(ga) > 1e-30 * fro2) {
+                    double cs, sn;
+                    jacobi_rotation(al, be, ga, cs, sn);
                     for (int r = sub; r < k; r += TPP) {
                         const double x = As[p][r], y = As[q][r];
                         As[p][r] = cs * x - sn * y; As[q][r] = sn * x + cs * y;
