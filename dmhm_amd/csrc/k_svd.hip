@@ -89,37 +89,59 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
     const int kp = k + (k & 1);
     constexpr int LOG = TPP == 8 ? 3 : (TPP == 4 ? 2 : (TPP == 2 ? 1 : 0));
     static_assert((1 << LOG) == TPP, "TPP must be 1, 2, 4 or 8");
+    constexpr int RPT = (KMAX + TPP - 1) / TPP;     // rows per thread
     const int pi = tid >> LOG, sub = tid & (TPP - 1);
+    __syncthreads();
+    const double froTol = 1e-30 * fro2;
+    // Round-robin order: seat 0 keeps its column, the others advance by one
+    // seat per round; pair pi takes seats pi and kp - 1 - pi. The column at a
+    // seat after `round` rounds is 1 + (seat - 1 + round) mod (kp - 1), kept
+    // here incrementally.
+    auto next = [&](int c) { return c == 0 ? 0 : (c == kp - 1 ? 1 : c + 1); };
     for (int sweep = 0; sweep < 60; ++sweep) {
         __syncthreads();
         if (tid == 0) rotated = 0;
         __syncthreads();
+        int a0 = pi, a1 = kp - 1 - pi;                  // round 0: column = seat
         for (int round = 0; round < kp - 1; ++round) {
             if (pi < kp / 2) {
-                auto pos = [&](int x) { return x == 0 ? 0 : 1 + ((x - 1 + round) % (kp - 1)); };
-                const int a0 = pos(pi), a1 = pos(kp - 1 - pi);
                 const int p = min(a0, a1), q = max(a0, a1);
-                double al = 0, be = 0, ga = 0;
-                if (q < k)
-                    for (int r = sub; r < k; r += TPP) {
-                        const double x = As[p][r], y = As[q][r];
-                        al += x * x; be += y * y; ga += x * y;
+                if (q < k) {
+                    // All of the group's LDS reads first, so their latencies overlap.
+                    double x[RPT], y[RPT], vx[RPT], vy[RPT];
+#pragma unroll
+                    for (int i = 0; i < RPT; ++i) {
+                        const int r = sub + i * TPP;
+                        const bool in = r < k;
+                        x[i] = in ? As[p][r] : 0.0;
+                        y[i] = in ? As[q][r] : 0.0;
+                        vx[i] = in ? Vs[p][r] : 0.0;
+                        vy[i] = in ? Vs[q][r] : 0.0;
                     }
-                al = group_sum<TPP>(al);
-                be = group_sum<TPP>(be);
-                ga = group_sum<TPP>(ga);
-                if (q < k && ga * ga > 1e-30 * (al * be) && fabs(ga) > 1e-30 * fro2) {
-                    double cs, sn;
-                    jacobi_rotation(al, be, ga, cs, sn);
-                    for (int r = sub; r < k; r += TPP) {
-                        const double x = As[p][r], y = As[q][r];
-                        As[p][r] = cs * x - sn * y; As[q][r] = sn * x + cs * y;
-                        const double vx = Vs[p][r], vy = Vs[q][r];
-                        Vs[p][r] = cs * vx - sn * vy; Vs[q][r] = sn * vx + cs * vy;
+                    double al = 0, be = 0, ga = 0;
+#pragma unroll
+                    for (int i = 0; i < RPT; ++i) {
+                        al += x[i] * x[i]; be += y[i] * y[i]; ga += x[i] * y[i];
                     }
-                    if (sub == 0) rotated = 1;
+                    al = group_sum<TPP>(al);
+                    be = group_sum<TPP>(be);
+                    ga = group_sum<TPP>(ga);
+                    if (ga * ga > 1e-30 * (al * be) && fabs(ga) > froTol) {
+                        double cs, sn;
+                        jacobi_rotation(al, be, ga, cs, sn);
+#pragma unroll
+                        for (int i = 0; i < RPT; ++i) {
+                            const int r = sub + i * TPP;
+                            if (r < k) {
+                                As[p][r] = cs * x[i] - sn * y[i]; As[q][r] = sn * x[i] + cs * y[i];
+                                Vs[p][r] = cs * vx[i] - sn * vy[i]; Vs[q][r] = sn * vx[i] + cs * vy[i];
+                            }
+                        }
+                        if (sub == 0) rotated = 1;
+                    }
                 }
             }
+            a0 = next(a0); a1 = next(a1);
             __syncthreads();
         }
         const int rot = rotated;
