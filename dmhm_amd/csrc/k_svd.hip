@@ -87,8 +87,8 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
         if ((tid & 63) == 0) atomicAdd(&fro2, s);
     }
     const int kp = k + (k & 1);
-    constexpr int LOG = TPP == 8 ? 3 : (TPP == 4 ? 2 : (TPP == 2 ? 1 : 0));
-    static_assert((1 << LOG) == TPP, "TPP must be 1, 2, 4 or 8");
+    constexpr int LOG = TPP == 16 ? 4 : (TPP == 8 ? 3 : (TPP == 4 ? 2 : (TPP == 2 ? 1 : 0)));
+    static_assert((1 << LOG) == TPP, "TPP must be 1, 2, 4, 8 or 16");
     constexpr int RPT = (KMAX + TPP - 1) / TPP;     // rows per thread
     const int pi = tid >> LOG, sub = tid & (TPP - 1);
     __syncthreads();
@@ -196,6 +196,8 @@ __global__ void __launch_bounds__(NT) svd_kernel(Bases b, const SvdTask* __restr
     }
 }
 
+constexpr int kSvdLatencyTasks = 256;
+
 template <typename T>
 void launch_svd_phase(const Phase& ph, const Bases& b, TruncPolicy pol, hipStream_t s)
 {
@@ -203,7 +205,12 @@ void launch_svd_phase(const Phase& ph, const Bases& b, TruncPolicy pol, hipStrea
     if (n == 0) return;
     const SvdTask* t = static_cast<const SvdTask*>(ph.dTasks);
     if (ph.svdK > 64) throw std::runtime_error("SVD factor wider than 64");
-    if (ph.svdK <= 24) svd_kernel<T, 24, 128><<<n, 128, 0, s>>>(b, t, pol.tolMode, pol.relTol);
+    // 24-column factors: 16 threads per column pair for phases of few factors
+    // (one workgroup per CU, the time is one factor's latency: 89 us against
+    // 99 us with 8 threads for 80 factors of k = 20), 8 threads beyond
+    // (120 us against 136 us for 850 factors).
+    if (ph.svdK <= 24 && n < kSvdLatencyTasks) svd_kernel<T, 24, 192, 16><<<n, 192, 0, s>>>(b, t, pol.tolMode, pol.relTol);
+    else if (ph.svdK <= 24) svd_kernel<T, 24, 128, 8><<<n, 128, 0, s>>>(b, t, pol.tolMode, pol.relTol);
     else if (ph.svdK <= 40) svd_kernel<T, 40, 192><<<n, 192, 0, s>>>(b, t, pol.tolMode, pol.relTol);
     else svd_kernel<T, 64, 256><<<n, 256, 0, s>>>(b, t, pol.tolMode, pol.relTol);
     HIP_LAUNCH_CHECK();
