@@ -501,7 +501,11 @@ def test_rounded_update_early_exit_concatenates():
     assert np.array_equal(U, Ur) and np.array_equal(V, Vr)
 
 
-@pytest.mark.parametrize("m,n,max_rank", [(50, 20, 8), (10, 30, 6), (20, 90, 5), (64, 64, 64)])
+# Tall panels reach every register class of the QR (m <= 64, 256, 512 and
+# 1024 rows; 24- and 40-column variants) and the wide fallback.
+@pytest.mark.parametrize("m,n,max_rank", [(50, 20, 8), (10, 30, 6), (20, 90, 5), (64, 64, 64), (63, 17, 17),
+                                          (200, 20, 12), (200, 36, 30), (400, 36, 20), (500, 23, 23),
+                                          (900, 20, 16), (1000, 24, 10)])
 def test_compress_matches_oracle(m, n, max_rank):
     rng = np.random.default_rng(m * n)
     D = rng.standard_normal((m, n)) * np.logspace(0, -6, n)
